@@ -1,0 +1,606 @@
+// precompute.hip — the six precompute passes of the reference as CDNA4
+// kernels (shaders/transmittance.comp, direct_irradiance.comp,
+// single_scattering.comp, scattering_density.comp, indirect_irradiance.comp,
+// multiple_scattering.comp), plus the irradiance clear.
+//
+// Every LUT is a row-major float4 array in HBM (x fastest).  Launch
+// wrappers at the bottom take a DevParams by value and raw device pointers;
+// the host runtime (runtime.hip) owns allocation and ordering.
+#include <math.h>
+
+#include "atmo_device.hpp"
+#include "kernels.hpp"
+
+namespace fbk {
+
+// ------------------------------------------------------------------ tables
+// Direction tables of the two spherical quadratures, computed once on the
+// host in binary32 exactly as the reference evaluates them
+// (scattering_density.comp:34-66, indirect_irradiance.comp:20-33), and in
+// binary64 for the precision mode's geometry.
+//   density:  l = 0..15 (theta), m = 0..31 (phi), dtheta = dphi = pi/16
+//   indirect: j = 0..15 (theta), i = 0..63 (phi), dtheta = dphi = pi/32
+__constant__ float4 c_den_dir[kDenTheta * kDenPhi];  // (cos phi sin t, sin phi sin t, cos t, domega)
+__constant__ float c_den_cos[kDenTheta];
+__constant__ double c_den_cos_d[kDenTheta];
+__constant__ float4 c_ind_dir[kIndTheta * kIndPhi];
+__constant__ float c_ind_cos[kIndTheta];
+__constant__ double c_ind_cos_d[kIndTheta];
+
+hipError_t upload_tables() {
+    static float4 den[kDenTheta * kDenPhi], ind[kIndTheta * kIndPhi];
+    static float dcos[kDenTheta], icos[kIndTheta];
+    static double dcos_d[kDenTheta], icos_d[kIndTheta];
+    const float PI = 3.14159265358979323846f;
+    {
+        const float dphi = PI / float(16), dtheta = PI / float(16);
+        const double dphi_d = M_PI / 16.0, dtheta_d = M_PI / 16.0;
+        for (int l = 0; l < kDenTheta; ++l) {
+            float theta = (float(l) + 0.5f) * dtheta;
+            float ct = cosf(theta), st = sinf(theta);
+            dcos[l] = ct;
+            dcos_d[l] = cos((double(l) + 0.5) * dtheta_d);
+            for (int m = 0; m < kDenPhi; ++m) {
+                float phi = (float(m) + 0.5f) * dphi;
+                float domega = dtheta * dphi * sinf(theta);
+                den[l * kDenPhi + m] = make_float4(cosf(phi) * st, sinf(phi) * st, ct, domega);
+            }
+            (void)dphi_d;
+        }
+    }
+    {
+        const float dphi = PI / float(32), dtheta = PI / float(32);
+        const double dtheta_d = M_PI / 32.0;
+        for (int j = 0; j < kIndTheta; ++j) {
+            float theta = (float(j) + 0.5f) * dtheta;
+            icos[j] = cosf(theta);
+            icos_d[j] = cos((double(j) + 0.5) * dtheta_d);
+            for (int i = 0; i < kIndPhi; ++i) {
+                float phi = (float(i) + 0.5f) * dphi;
+                float domega = dtheta * dphi * sinf(theta);
+                ind[j * kIndPhi + i] =
+                    make_float4(cosf(phi) * sinf(theta), sinf(phi) * sinf(theta), cosf(theta), domega);
+            }
+        }
+    }
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_den_dir), den, sizeof(den))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_den_cos), dcos, sizeof(dcos))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_den_cos_d), dcos_d, sizeof(dcos_d))) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ind_dir), ind, sizeof(ind))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ind_cos), icos, sizeof(icos))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ind_cos_d), icos_d, sizeof(icos_d))) != hipSuccess)
+        return e;
+    return hipSuccess;
+}
+
+template <typename G>
+__device__ __forceinline__ G den_cos(int l);
+template <>
+__device__ __forceinline__ float den_cos<float>(int l) { return c_den_cos[l]; }
+template <>
+__device__ __forceinline__ double den_cos<double>(int l) { return c_den_cos_d[l]; }
+template <typename G>
+__device__ __forceinline__ G ind_cos(int j);
+template <>
+__device__ __forceinline__ float ind_cos<float>(int j) { return c_ind_cos[j]; }
+template <>
+__device__ __forceinline__ double ind_cos<double>(int j) { return c_ind_cos_d[j]; }
+
+// Running sum with optional Neumaier compensation (precision mode).
+template <bool COMP>
+struct Sum {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float x) {
+        if (COMP) {
+            float t = s + x;
+            if (fabsf(s) >= fabsf(x))
+                c += (s - t) + x;
+            else
+                c += (x - t) + s;
+            s = t;
+        } else {
+            s = s + x;
+        }
+    }
+    __device__ __forceinline__ float value() const { return COMP ? s + c : s; }
+};
+
+// ============================================================ 1. transmittance
+// transmittance.comp:8-46,71-80.  One thread per texel; the three optical
+// lengths share the sample positions (the reference recomputes r_i per
+// profile with identical inputs).
+template <typename G, bool COMP>
+__global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= P.t_mu || j >= P.t_r) return;
+    G x_mu = G(i) / G(P.t_mu - 1);
+    G x_r = G(j) / G(P.t_r - 1);
+    G r, mu;
+    rmu_from_transmittance_unit<G>(P, x_mu, x_r, r, mu);
+    const int SAMPLE_COUNT = 500;
+    G dx = distance_to_top<G>(P, r, mu) / G(SAMPLE_COUNT);
+    float dxf = float(dx);
+    Sum<COMP> lr, lm, lo;
+    for (int k = 0; k <= SAMPLE_COUNT; ++k) {
+        G d_i = G(k) * dx;
+        G r_i = sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r);
+        float alt = float(r_i - G(P.bottom));
+        float w = (k == 0 || k == SAMPLE_COUNT) ? 0.5f : 1.0f;
+        lr.add(profile_density(P.prof[0], alt) * w * dxf);
+        lm.add(profile_density(P.prof[1], alt) * w * dxf);
+        lo.add(profile_density(P.prof[2], alt) * w * dxf);
+    }
+    float LR = lr.value(), LM = lm.value(), LO = lo.value();
+    float4 out;
+    out.x = expf(-(P.rayleigh_scattering[0] * LR + P.mie_extinction[0] * LM +
+                   P.absorption_extinction[0] * LO));
+    out.y = expf(-(P.rayleigh_scattering[1] * LR + P.mie_extinction[1] * LM +
+                   P.absorption_extinction[1] * LO));
+    out.z = expf(-(P.rayleigh_scattering[2] * LR + P.mie_extinction[2] * LM +
+                   P.absorption_extinction[2] * LO));
+    out.w = 1.0f;
+    T[j * P.t_mu + i] = out;
+}
+
+// ====================================================== 2. direct irradiance
+// direct_irradiance.comp:10-46
+template <typename G>
+__global__ __launch_bounds__(256) void k_direct_irradiance(DevParams P,
+                                                           const float4* __restrict__ T,
+                                                           float4* __restrict__ dE) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= P.i_mus || j >= P.i_r) return;
+    G x_mu_s = G(i) / G(P.i_mus - 1);
+    G x_r = G(j) / G(P.i_r - 1);
+    G r, mu_s;
+    rmus_from_irradiance_unit<G>(P, x_mu_s, x_r, r, mu_s);
+    G alpha_s = G(P.sun_angular_radius);
+    G avg = mu_s < -alpha_s
+                ? G(0)
+                : (mu_s > alpha_s ? mu_s : (mu_s + alpha_s) * (mu_s + alpha_s) / (G(4) * alpha_s));
+    float a = float(avg);
+    float3 t = transmittance_to_top<G>(P, T, r, mu_s);
+    dE[j * P.i_mus + i] =
+        make_float4(P.solar[0] * t.x * a, P.solar[1] * t.y * a, P.solar[2] * t.z * a, 0.0f);
+}
+
+// ===================================================== 3. single scattering
+// single_scattering.comp:10-101.  One thread per texel, x fastest, so the
+// three LUT stores of a wave are contiguous.
+template <typename G, bool COMP>
+__global__ __launch_bounds__(256) void k_single_scattering(DevParams P,
+                                                           const float4* __restrict__ T,
+                                                           float4* __restrict__ dR,
+                                                           float4* __restrict__ dM,
+                                                           float4* __restrict__ S) {
+    const unsigned W = unsigned(P.s_nu * P.s_mus);
+    const size_t n = size_t(W) * P.s_mu * P.s_r;
+    size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    unsigned x = unsigned(idx % W), y = unsigned((idx / W) % P.s_mu),
+             z = unsigned(idx / (size_t(W) * P.s_mu));
+    TexelCoord<G> c = scattering_texel_coord<G>(P, x, y, z);
+    const int SAMPLE_COUNT = 50;
+    G dx = distance_to_nearest<G>(P, c.r, c.mu, c.ground) / G(SAMPLE_COUNT);
+    Sum<COMP> rs[3], ms[3];
+    for (int i = 0; i <= SAMPLE_COUNT; ++i) {
+        G d_i = G(i) * dx;
+        G r_d = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * c.r * c.mu * d_i + c.r * c.r));
+        G mu_s_d = clamp_cosine<G>((c.r * c.mu_s + d_i * c.nu) / r_d);
+        float3 tv = get_transmittance<G>(P, T, c.r, c.mu, d_i, c.ground);
+        float3 ts = transmittance_to_sun<G>(P, T, r_d, mu_s_d);
+        float alt = float(r_d - G(P.bottom));
+        float rd = profile_density(P.prof[0], alt);
+        float md = profile_density(P.prof[1], alt);
+        float w = (i == 0 || i == SAMPLE_COUNT) ? 0.5f : 1.0f;
+        float t0 = tv.x * ts.x, t1 = tv.y * ts.y, t2 = tv.z * ts.z;
+        rs[0].add(t0 * rd * w);
+        rs[1].add(t1 * rd * w);
+        rs[2].add(t2 * rd * w);
+        ms[0].add(t0 * md * w);
+        ms[1].add(t1 * md * w);
+        ms[2].add(t2 * md * w);
+    }
+    float dxf = float(dx);
+    float3 ray = make_float3(rs[0].value() * dxf * P.solar[0] * P.rayleigh_scattering[0],
+                             rs[1].value() * dxf * P.solar[1] * P.rayleigh_scattering[1],
+                             rs[2].value() * dxf * P.solar[2] * P.rayleigh_scattering[2]);
+    float3 mie = make_float3(ms[0].value() * dxf * P.solar[0] * P.mie_scattering[0],
+                             ms[1].value() * dxf * P.solar[1] * P.mie_scattering[1],
+                             ms[2].value() * dxf * P.solar[2] * P.mie_scattering[2]);
+    dR[idx] = make_float4(ray.x, ray.y, ray.z, 0.0f);
+    dM[idx] = make_float4(mie.x, mie.y, mie.z, 0.0f);
+    S[idx] = make_float4(ray.x, ray.y, ray.z, mie.x);
+}
+
+// ===================================================== 4. scattering density
+// scattering_density.comp:10-155.
+//
+// For a texel (r, mu, mu_s, nu) every incident-radiance lookup of the
+// 16x32-direction quadrature is GetScattering(r, cos(theta_l), mu_s, nu1),
+// so its (u_r, u_mu, u_mu_s) coordinates depend only on the texel's r and
+// mu_s and on theta_l; only u_nu varies with phi.  All MU*NU texels that
+// share (r, mu_s) therefore read the same 16 "nu profiles": for each theta
+// row, the trilinear (mu_s, mu, r) sample of every nu slice k = -1..NU
+// (slices -1 and NU reproduce CLAMP_TO_EDGE at the LUT's x edges).  A block
+// builds those profiles in LDS once; each sample is then a 1D lerp between
+// two LDS entries — the same value the reference's two trilinear texture
+// fetches produce, with 8x-64x fewer gathers.
+//
+// Grid: (ceil(MU*NU/256), MUS, R); each block serves one (r, mu_s) tile.
+template <typename G>
+__device__ __forceinline__ float4 trilinear_slice(const DevParams& P,
+                                                  const float4* __restrict__ tex, int slice,
+                                                  G u_mu_s, const Taps& ty, const Taps& tz) {
+    const int W = P.s_nu * P.s_mus;
+    G x = (G(slice) + u_mu_s) / G(P.s_nu);
+    Taps tx = linear_taps<G>(x, W);
+    const float4* p00 = tex + (size_t(tz.i0) * P.s_mu + ty.i0) * W;
+    const float4* p01 = tex + (size_t(tz.i0) * P.s_mu + ty.i1) * W;
+    const float4* p10 = tex + (size_t(tz.i1) * P.s_mu + ty.i0) * W;
+    const float4* p11 = tex + (size_t(tz.i1) * P.s_mu + ty.i1) * W;
+    return lerp4(lerp4(lerp4(ldg4(p00 + tx.i0), ldg4(p00 + tx.i1), tx.w),
+                       lerp4(ldg4(p01 + tx.i0), ldg4(p01 + tx.i1), tx.w), ty.w),
+                 lerp4(lerp4(ldg4(p10 + tx.i0), ldg4(p10 + tx.i1), tx.w),
+                       lerp4(ldg4(p11 + tx.i0), ldg4(p11 + tx.i1), tx.w), ty.w),
+                 tz.w);
+}
+
+// Lerp of a nu profile (LDS, slots -1..NU stored at 0..NU+1) at nu.
+__device__ __forceinline__ float4 profile_lookup(const float4* prof, int nu_size, float nu) {
+    float u_nu = (nu + 1.0f) / 2.0f;
+    float tex_coord_x = u_nu * float(nu_size - 1);
+    float tex_x = floorf(tex_coord_x);
+    float lerp = tex_coord_x - tex_x;
+    tex_x = fminf(fmaxf(tex_x, -1.0f), float(nu_size - 1));
+    int e = int(tex_x) + 1;
+    float4 a = prof[e], b = prof[e + 1];
+    return make_float4(a.x * (1.0f - lerp) + b.x * lerp, a.y * (1.0f - lerp) + b.y * lerp,
+                       a.z * (1.0f - lerp) + b.z * lerp, a.w * (1.0f - lerp) + b.w * lerp);
+}
+
+template <typename G>
+__global__ __launch_bounds__(256) void k_scattering_density(
+    DevParams P, int scattering_order, const float4* __restrict__ T,
+    const float4* __restrict__ dR, const float4* __restrict__ dM,
+    const float4* __restrict__ dMS, const float4* __restrict__ dE, float4* __restrict__ J) {
+    // Dynamic LDS: [ntex][16][NU+2] profile slots, then the I_MUS-texel
+    // delta_irradiance row at r = bottom.
+    extern __shared__ float4 s_dyn[];
+    __shared__ float4 s_ground[kDenTheta];  // (T_g*albedo/pi).rgb, distance_to_ground
+    __shared__ int s_ground_mask;
+
+    const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
+    const int W = NU * MUS;
+    const int j = blockIdx.y;  // mu_s index within a nu slice
+    const int z = blockIdx.z;  // r index
+    const int tid = threadIdx.x;
+    const bool single = (scattering_order - 1) == 1;
+    const int nslots = NU + 2;
+    float4* s_prof = s_dyn;                                      // [tex][l][slot]
+    float4* s_erow = s_dyn + (single ? 2 : 1) * kDenTheta * nslots;
+
+    // Tile coordinates: r and mu_s shared by the tile (texel x = j, y = 0).
+    TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
+    const G r = tc.r;
+    const G u_r = scattering_u_r<G>(P, r);
+    const G u_mu_s = scattering_u_mu_s<G>(P, tc.mu_s);
+    const Taps tz = linear_taps<G>(u_r, P.s_r);
+
+    // Profiles: 16 theta rows x (NU+2) slots x (2 textures at order 1).
+    const int nprof = kDenTheta * nslots * (single ? 2 : 1);
+    for (int e = tid; e < nprof; e += blockDim.x) {
+        int tex_sel = e / (kDenTheta * nslots);
+        int rem = e % (kDenTheta * nslots);
+        int l = rem / nslots, slot = rem % nslots;
+        G ct = den_cos<G>(l);
+        bool g_l = ray_intersects_ground<G>(P, r, ct);
+        G u_mu = scattering_u_mu<G>(P, r, ct, g_l);
+        Taps ty = linear_taps<G>(u_mu, MU);
+        const float4* tex = single ? (tex_sel == 0 ? dR : dM) : dMS;
+        s_prof[(tex_sel * kDenTheta + l) * nslots + slot] =
+            trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
+    }
+    // Ground terms per theta row (scattering_density.comp:45-60).
+    if (tid < kDenTheta) {
+        int l = tid;
+        G ct = den_cos<G>(l);
+        bool g_l = ray_intersects_ground<G>(P, r, ct);
+        float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (g_l) {
+            G dg = distance_to_bottom<G>(P, r, ct);
+            float3 tg = get_transmittance<G>(P, T, r, ct, dg, true);
+            gv = make_float4(tg.x * P.albedo[0] * (1.0f / kPi), tg.y * P.albedo[1] * (1.0f / kPi),
+                             tg.z * P.albedo[2] * (1.0f / kPi), float(dg));
+        }
+        s_ground[l] = gv;
+        unsigned long long b = __ballot(g_l);
+        if (l == 0) s_ground_mask = int(b & 0xffffull);
+    }
+    // delta_irradiance row at r = bottom (irradiance.h:20-30 with x_r = 0).
+    {
+        G uy = tex_from_unit<G>(G(0), P.i_r);
+        Taps ty = linear_taps<G>(uy, P.i_r);
+        for (int x = tid; x < P.i_mus; x += blockDim.x)
+            s_erow[x] = lerp4(ldg4(dE + ty.i0 * P.i_mus + x), ldg4(dE + ty.i1 * P.i_mus + x), ty.w);
+    }
+    __syncthreads();
+
+    const int t = blockIdx.x * blockDim.x + tid;  // texel within the tile
+    if (t >= MU * NU) return;
+    const int k = t % NU, y = t / NU;
+    const unsigned x = unsigned(k * MUS + j);
+    TexelCoord<G> c = scattering_texel_coord<G>(P, x, unsigned(y), unsigned(z));
+
+    // omega, omega_s (scattering_density.comp:28-32), built in G.
+    G om_x = sqrt(G(1) - c.mu * c.mu);
+    G sdx = om_x == G(0) ? G(0) : (c.nu - c.mu * c.mu_s) / om_x;
+    G sdy = sqrt(fmax(G(1) - sdx * sdx - c.mu_s * c.mu_s, G(0)));
+    const float omx = float(om_x), omz = float(c.mu);
+    const float osx = float(sdx), osy = float(sdy), osz = float(c.mu_s);
+    const float rf = float(c.r);
+    const float alt = float(c.r - G(P.bottom));
+    const float rho_r = profile_density(P.prof[0], alt);
+    const float rho_m = profile_density(P.prof[1], alt);
+    const float cr0 = P.rayleigh_scattering[0] * rho_r, cr1 = P.rayleigh_scattering[1] * rho_r,
+                cr2 = P.rayleigh_scattering[2] * rho_r;
+    const float cm0 = P.mie_scattering[0] * rho_m, cm1 = P.mie_scattering[1] * rho_m,
+                cm2 = P.mie_scattering[2] * rho_m;
+    const int gmask = s_ground_mask;
+    const float g = P.g;
+
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int l = 0; l < kDenTheta; ++l) {
+        const float4* pr = s_prof + l * nslots;
+        const float4* pm = s_prof + (kDenTheta + l) * nslots;
+        const bool ground_l = (gmask >> l) & 1;
+        const float4 gv = s_ground[l];
+        for (int m = 0; m < kDenPhi; ++m) {
+            const float4 oi = c_den_dir[l * kDenPhi + m];
+            float nu1 = osx * oi.x + osy * oi.y + osz * oi.z;
+            float i0, i1, i2;
+            if (single) {
+                float4 vr = profile_lookup(pr, NU, nu1);
+                float4 vm = profile_lookup(pm, NU, nu1);
+                float prr = phase_rayleigh(nu1), pmm = phase_mie(g, nu1);
+                i0 = vr.x * prr + vm.x * pmm;
+                i1 = vr.y * prr + vm.y * pmm;
+                i2 = vr.z * prr + vm.z * pmm;
+            } else {
+                float4 v = profile_lookup(pr, NU, nu1);
+                i0 = v.x;
+                i1 = v.y;
+                i2 = v.z;
+            }
+            if (ground_l) {
+                float gx = oi.x * gv.w, gy = oi.y * gv.w, gz = rf + oi.z * gv.w;
+                float len = sqrtf(gx * gx + gy * gy + gz * gz);
+                gx = gx / len;
+                gy = gy / len;
+                gz = gz / len;
+                float gmu = gx * osx + gy * osy + gz * osz;
+                float u = tex_from_unit<float>(gmu * 0.5f + 0.5f, P.i_mus);
+                Taps tx = linear_taps<float>(u, P.i_mus);
+                float4 e = lerp4(s_erow[tx.i0], s_erow[tx.i1], tx.w);
+                i0 += gv.x * e.x;
+                i1 += gv.y * e.y;
+                i2 += gv.z * e.z;
+            }
+            float nu2 = omx * oi.x + 0.0f * oi.y + omz * oi.z;
+            float prr = phase_rayleigh(nu2), pmm = phase_mie(g, nu2);
+            a0 += i0 * (cr0 * prr + cm0 * pmm) * oi.w;
+            a1 += i1 * (cr1 * prr + cm1 * pmm) * oi.w;
+            a2 += i2 * (cr2 * prr + cm2 * pmm) * oi.w;
+        }
+    }
+    J[(size_t(z) * MU + y) * W + x] = make_float4(a0, a1, a2, 0.0f);
+}
+
+// =================================================== 5. indirect irradiance
+// indirect_irradiance.comp:10-74.  One wavefront per irradiance texel: lane
+// i takes phi_i (64 lanes = the 64 phi samples), loops the 16 theta rows,
+// and the wave reduces.  As in the density pass, the lookups of one theta
+// row share (u_r, u_mu, u_mu_s), so each wave first builds its 16 nu
+// profiles in LDS.
+constexpr int kIndWaves = 4;
+
+template <typename G>
+__global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
+    DevParams P, int scattering_order, const float4* __restrict__ dR,
+    const float4* __restrict__ dM, const float4* __restrict__ dMS, float4* __restrict__ dE,
+    float4* __restrict__ E) {
+    // Dynamic LDS: per wave [ntex][16][NU+2] profile slots.
+    extern __shared__ float4 s_dyn[];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int texel = blockIdx.x * kIndWaves + wave;
+    const int n = P.i_mus * P.i_r;
+    const bool active = texel < n;
+    const int ti = active ? texel % P.i_mus : 0, tj = active ? texel / P.i_mus : 0;
+    const bool single = scattering_order == 1;
+    const int NU = P.s_nu, nslots = NU + 2;
+    float4* s_prof = s_dyn + wave * (single ? 2 : 1) * kIndTheta * nslots;
+
+    G x_mu_s = G(ti) / G(P.i_mus - 1);
+    G x_r = G(tj) / G(P.i_r - 1);
+    G r, mu_s;
+    rmus_from_irradiance_unit<G>(P, x_mu_s, x_r, r, mu_s);
+    const G u_r = scattering_u_r<G>(P, r);
+    const G u_mu_s = scattering_u_mu_s<G>(P, mu_s);
+    const Taps tz = linear_taps<G>(u_r, P.s_r);
+    const int nprof = kIndTheta * nslots * (single ? 2 : 1);
+    for (int e = lane; e < nprof; e += 64) {
+        int tex_sel = e / (kIndTheta * nslots);
+        int rem = e % (kIndTheta * nslots);
+        int jj = rem / nslots, slot = rem % nslots;
+        G ct = ind_cos<G>(jj);
+        G u_mu = scattering_u_mu<G>(P, r, ct, false);
+        Taps ty = linear_taps<G>(u_mu, P.s_mu);
+        const float4* tex = single ? (tex_sel == 0 ? dR : dM) : dMS;
+        s_prof[(tex_sel * kIndTheta + jj) * nslots + slot] =
+            trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
+    }
+    __syncthreads();
+    if (!active) return;
+
+    const float osx = float(sqrt(G(1) - mu_s * mu_s)), osz = float(mu_s);
+    const float g = P.g;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int jj = 0; jj < kIndTheta; ++jj) {
+        const float4 om = c_ind_dir[jj * kIndPhi + lane];
+        float nu = om.x * osx + om.y * 0.0f + om.z * osz;
+        float s0, s1, s2;
+        if (single) {
+            float4 vr = profile_lookup(s_prof + jj * nslots, NU, nu);
+            float4 vm = profile_lookup(s_prof + (kIndTheta + jj) * nslots, NU, nu);
+            float prr = phase_rayleigh(nu), pmm = phase_mie(g, nu);
+            s0 = vr.x * prr + vm.x * pmm;
+            s1 = vr.y * prr + vm.y * pmm;
+            s2 = vr.z * prr + vm.z * pmm;
+        } else {
+            float4 v = profile_lookup(s_prof + jj * nslots, NU, nu);
+            s0 = v.x;
+            s1 = v.y;
+            s2 = v.z;
+        }
+        a0 += s0 * om.z * om.w;
+        a1 += s1 * om.z * om.w;
+        a2 += s2 * om.z * om.w;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        a1 += __shfl_xor(a1, off, 64);
+        a2 += __shfl_xor(a2, off, 64);
+    }
+    if (lane == 0) {
+        int o = tj * P.i_mus + ti;
+        dE[o] = make_float4(a0, a1, a2, 0.0f);
+        float4 prev = E[o];
+        E[o] = make_float4(a0 + prev.x, a1 + prev.y, a2 + prev.z, 0.0f + prev.w);
+    }
+}
+
+// =================================================== 6. multiple scattering
+// multiple_scattering.comp:9-93.  One thread per texel.
+template <typename G, bool COMP>
+__global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
+                                                             const float4* __restrict__ T,
+                                                             const float4* __restrict__ J,
+                                                             float4* __restrict__ dMS,
+                                                             float4* __restrict__ S) {
+    const unsigned W = unsigned(P.s_nu * P.s_mus);
+    const size_t n = size_t(W) * P.s_mu * P.s_r;
+    size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    unsigned x = unsigned(idx % W), y = unsigned((idx / W) % P.s_mu),
+             z = unsigned(idx / (size_t(W) * P.s_mu));
+    TexelCoord<G> c = scattering_texel_coord<G>(P, x, y, z);
+    const int SAMPLE_COUNT = 50;
+    G dx = distance_to_nearest<G>(P, c.r, c.mu, c.ground) / G(SAMPLE_COUNT);
+    float dxf = float(dx);
+    Sum<COMP> s0, s1, s2;
+    for (int i = 0; i <= SAMPLE_COUNT; ++i) {
+        G d_i = G(i) * dx;
+        G r_i = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * c.r * c.mu * d_i + c.r * c.r));
+        G mu_i = clamp_cosine<G>((c.r * c.mu + d_i) / r_i);
+        G mu_s_i = clamp_cosine<G>((c.r * c.mu_s + d_i * c.nu) / r_i);
+        float4 sv = get_scattering_tex<G>(P, J, r_i, mu_i, mu_s_i, c.nu, c.ground);
+        float3 tv = get_transmittance<G>(P, T, c.r, c.mu, d_i, c.ground);
+        float w = (i == 0 || i == SAMPLE_COUNT) ? 0.5f : 1.0f;
+        s0.add(sv.x * tv.x * dxf * w);
+        s1.add(sv.y * tv.y * dxf * w);
+        s2.add(sv.z * tv.z * dxf * w);
+    }
+    float m0 = s0.value(), m1 = s1.value(), m2 = s2.value();
+    dMS[idx] = make_float4(m0, m1, m2, 0.0f);
+    float pr = phase_rayleigh(float(c.nu));
+    float4 prev = S[idx];
+    S[idx] = make_float4(m0 / pr + prev.x, m1 / pr + prev.y, m2 / pr + prev.z, 0.0f + prev.w);
+}
+
+// ================================================================== clear
+__global__ void k_fill_zero(float4* __restrict__ p, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ========================================================== launch wrappers
+static inline unsigned cdiv(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
+
+hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s) {
+    dim3 grid(cdiv(P.t_mu, 64), P.t_r);
+    if (P.flags & 1)
+        k_transmittance<double, true><<<grid, 64, 0, s>>>(P, T);
+    else
+        k_transmittance<float, false><<<grid, 64, 0, s>>>(P, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,
+                                    hipStream_t s) {
+    dim3 grid(cdiv(P.i_mus, 64), P.i_r);
+    if (P.flags & 1)
+        k_direct_irradiance<double><<<grid, 64, 0, s>>>(P, T, dE);
+    else
+        k_direct_irradiance<float><<<grid, 64, 0, s>>>(P, T, dE);
+    return hipGetLastError();
+}
+
+hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
+                                    float4* dM, float4* S, hipStream_t s) {
+    size_t n = size_t(P.s_nu) * P.s_mus * P.s_mu * P.s_r;
+    if (P.flags & 1)
+        k_single_scattering<double, true><<<cdiv(n, 256), 256, 0, s>>>(P, T, dR, dM, S);
+    else
+        k_single_scattering<float, false><<<cdiv(n, 256), 256, 0, s>>>(P, T, dR, dM, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
+                                     const float4* dR, const float4* dM, const float4* dMS,
+                                     const float4* dE, float4* J, hipStream_t s) {
+    dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, 256), P.s_mus, P.s_r);
+    const int ntex = (order - 1) == 1 ? 2 : 1;
+    size_t lds = sizeof(float4) * (size_t(ntex) * kDenTheta * (P.s_nu + 2) + P.i_mus);
+    if (P.flags & 1)
+        k_scattering_density<double><<<grid, 256, lds, s>>>(P, order, T, dR, dM, dMS, dE, J);
+    else
+        k_scattering_density<float><<<grid, 256, lds, s>>>(P, order, T, dR, dM, dMS, dE, J);
+    return hipGetLastError();
+}
+
+hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float4* dR,
+                                      const float4* dM, const float4* dMS, float4* dE, float4* E,
+                                      hipStream_t s) {
+    size_t n = size_t(P.i_mus) * P.i_r;
+    const int ntex = order == 1 ? 2 : 1;
+    size_t lds = sizeof(float4) * size_t(kIndWaves) * ntex * kIndTheta * (P.s_nu + 2);
+    if (P.flags & 1)
+        k_indirect_irradiance<double>
+            <<<cdiv(n, kIndWaves), 64 * kIndWaves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
+    else
+        k_indirect_irradiance<float>
+            <<<cdiv(n, kIndWaves), 64 * kIndWaves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
+    return hipGetLastError();
+}
+
+hipError_t launch_multiple_scattering(const DevParams& P, const float4* T, const float4* J,
+                                      float4* dMS, float4* S, hipStream_t s) {
+    size_t n = size_t(P.s_nu) * P.s_mus * P.s_mu * P.s_r;
+    if (P.flags & 1)
+        k_multiple_scattering<double, true><<<cdiv(n, 256), 256, 0, s>>>(P, T, J, dMS, S);
+    else
+        k_multiple_scattering<float, false><<<cdiv(n, 256), 256, 0, s>>>(P, T, J, dMS, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(float4* p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_fill_zero<<<cdiv(n, 256), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace fbk

@@ -1,0 +1,377 @@
+// render.hip — per-pixel sky and lighting lookups from a resident LUT set:
+// shaders/render_sky.h (GetSkyRadiance, GetSkyRadianceToPoint),
+// render_sky.frag (unprojection, dual-source output) and render_lighting.h
+// (GetSunAndSkyIrradiance), as compute kernels that index pixels directly
+// (the full-screen triangle of shaders/fullscreen.vert is not needed).
+#include <string.h>
+
+#include "atmo_device.hpp"
+#include "runtime.hpp"
+
+namespace fbk {
+
+__device__ __forceinline__ float dot3(float3 a, float3 b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// render_sky.h:9-19
+__device__ __forceinline__ float3 extrapolated_single_mie(const DevParams& P, float4 s) {
+    if (s.x <= 0.0f) return make_float3(0.0f, 0.0f, 0.0f);
+    float k = P.rayleigh_scattering[0] / P.mie_scattering[0];
+    return make_float3(s.x * s.w / s.x * k * (P.mie_scattering[0] / P.rayleigh_scattering[0]),
+                       s.y * s.w / s.x * k * (P.mie_scattering[1] / P.rayleigh_scattering[1]),
+                       s.z * s.w / s.x * k * (P.mie_scattering[2] / P.rayleigh_scattering[2]));
+}
+
+// render_sky.h:21-43 (GLSL mix of the two nu slices)
+__device__ __forceinline__ float4 combined_scattering(const DevParams& P,
+                                                      const float4* __restrict__ S, float r,
+                                                      float mu, float mu_s, float nu, bool ground,
+                                                      float3& mie) {
+    float4 c = get_scattering_tex<float>(P, S, r, mu, mu_s, nu, ground);
+    mie = extrapolated_single_mie(P, c);
+    return c;
+}
+
+// render_sky.h:45-109
+__device__ void sky_radiance(const DevParams& P, const float4* __restrict__ T,
+                             const float4* __restrict__ S, float3 camera, float3 view,
+                             float3 sun, float3& rad, float3& trans) {
+    float r = sqrtf(dot3(camera, camera));
+    float rmu = dot3(camera, view);
+    float dtop = -rmu - sqrtf(rmu * rmu - r * r + P.top * P.top);
+    if (dtop > 0.0f) {
+        camera = make_float3(camera.x + view.x * dtop, camera.y + view.y * dtop,
+                             camera.z + view.z * dtop);
+        r = P.top;
+        rmu += dtop;
+    } else if (r > P.top) {
+        trans = make_float3(1.0f, 1.0f, 1.0f);
+        rad = make_float3(0.0f, 0.0f, 0.0f);
+        return;
+    }
+    float mu = rmu / r;
+    float mu_s = dot3(camera, sun) / r;
+    float nu = dot3(view, sun);
+    bool ground = ray_intersects_ground<float>(P, r, mu);
+    trans = ground ? make_float3(0.0f, 0.0f, 0.0f) : transmittance_to_top<float>(P, T, r, mu);
+    float3 mie;
+    float4 sc = combined_scattering(P, S, r, mu, mu_s, nu, ground, mie);
+    float pr = phase_rayleigh(nu), pm = phase_mie(P.g, nu);
+    rad = make_float3(sc.x * pr + mie.x * pm, sc.y * pr + mie.y * pm, sc.z * pr + mie.z * pm);
+}
+
+// render_sky.h:111-191
+__device__ void sky_radiance_to_point(const DevParams& P, const float4* __restrict__ T,
+                                      const float4* __restrict__ S, float3 camera, float3 view,
+                                      float3 point, float3 sun, float3& rad, float3& trans) {
+    float r = sqrtf(dot3(camera, camera));
+    float rmu = dot3(camera, view);
+    float dtop = -rmu - sqrtf(rmu * rmu - r * r + P.top * P.top);
+    if (dtop > 0.0f) {
+        camera = make_float3(camera.x + view.x * dtop, camera.y + view.y * dtop,
+                             camera.z + view.z * dtop);
+        r = P.top;
+        rmu += dtop;
+    } else if (r > P.top) {
+        trans = make_float3(1.0f, 1.0f, 1.0f);
+        rad = make_float3(0.0f, 0.0f, 0.0f);
+        return;
+    }
+    float mu = rmu / r;
+    float mu_s = dot3(camera, sun) / r;
+    float nu = dot3(view, sun);
+    float3 pc = make_float3(point.x - camera.x, point.y - camera.y, point.z - camera.z);
+    float d = sqrtf(dot3(pc, pc));
+    bool ground = ray_intersects_ground<float>(P, r, mu);
+    trans = get_transmittance<float>(P, T, r, mu, d, ground);
+    float3 mie;
+    float4 sc = combined_scattering(P, S, r, mu, mu_s, nu, ground, mie);
+    if (!isinf(d)) {
+        float r_p = clamp_radius<float>(P, sqrtf(d * d + 2.0f * r * mu * d + r * r));
+        float mu_p = (r * mu + d) / r_p;
+        float mu_s_p = (r * mu_s + d * nu) / r_p;
+        float3 mie_p;
+        float4 sc_p = combined_scattering(P, S, r_p, mu_p, mu_s_p, nu, ground, mie_p);
+        sc.x = sc.x - trans.x * sc_p.x;
+        sc.y = sc.y - trans.y * sc_p.y;
+        sc.z = sc.z - trans.z * sc_p.z;
+        float mr = mie.x - trans.x * mie_p.x;
+        mie = extrapolated_single_mie(P, make_float4(sc.x, sc.y, sc.z, mr));
+        float h = smoothstep_g<float>(0.0f, 0.01f, mu_s);
+        mie = make_float3(mie.x * h, mie.y * h, mie.z * h);
+    }
+    float pr = phase_rayleigh(nu), pm = phase_mie(P.g, nu);
+    rad = make_float3(sc.x * pr + mie.x * pm, sc.y * pr + mie.y * pm, sc.z * pr + mie.z * pm);
+}
+
+struct DrawArgs {
+    float ivp[4][4];  // [column][row]
+    float cam[3];
+    float sun[3];
+};
+
+// render_sky.frag:24-35 for one pixel; fullscreen.vert:5-8 interpolates
+// screen_coords to ((px+0.5)/w, (py+0.5)/h) at pixel centres.
+__global__ __launch_bounds__(256) void k_render_sky(DevParams P, const float4* __restrict__ T,
+                                                    const float4* __restrict__ S, DrawArgs D,
+                                                    const float* __restrict__ depth, int w, int h,
+                                                    float4* __restrict__ color,
+                                                    float4* __restrict__ trans_out,
+                                                    int composite) {
+    int px = blockIdx.x * 16 + (threadIdx.x % 16);
+    int py = blockIdx.y * 16 + (threadIdx.x / 16);
+    if (px >= w || py >= h) return;
+    size_t idx = size_t(py) * w + px;
+    float sx = (float(px) + 0.5f) / float(w), sy = (float(py) + 0.5f) / float(h);
+    float n0 = 2.0f * sx - 1.0f, n1 = 2.0f * sy - 1.0f;
+    float v[4];
+    for (int row = 0; row < 4; ++row)
+        v[row] = D.ivp[0][row] * n0 + D.ivp[1][row] * n1 + D.ivp[2][row] * 0.0f + D.ivp[3][row];
+    float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    float3 view = make_float3(v[0] / len, v[1] / len, v[2] / len);
+    float dz = depth ? depth[idx] : 0.0f;
+    float wp[4];
+    for (int row = 0; row < 4; ++row)
+        wp[row] = D.ivp[0][row] * n0 + D.ivp[1][row] * n1 + D.ivp[2][row] * dz + D.ivp[3][row];
+    float3 world = make_float3(wp[0] / wp[3] * 1e-3f, wp[1] / wp[3] * 1e-3f, wp[2] / wp[3] * 1e-3f);
+    float3 rad, tr;
+    sky_radiance_to_point(P, T, S, make_float3(D.cam[0], D.cam[1], D.cam[2]), view, world,
+                          make_float3(D.sun[0], D.sun[1], D.sun[2]), rad, tr);
+    if (composite) {
+        // Dual-source blend dst = src0 + dst * src1 (src/render.rs:124-136);
+        // alpha: dst.a kept (src factor ZERO, dst factor ONE).
+        float4 dst = color[idx];
+        color[idx] = make_float4(rad.x + dst.x * tr.x, rad.y + dst.y * tr.y, rad.z + dst.z * tr.z,
+                                 dst.w);
+    } else {
+        color[idx] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    }
+    if (trans_out) trans_out[idx] = make_float4(tr.x, tr.y, tr.z, 1.0f);
+}
+
+__global__ __launch_bounds__(256) void k_render_rays(DevParams P, const float4* __restrict__ T,
+                                                     const float4* __restrict__ S,
+                                                     const float* __restrict__ cams, int cstride,
+                                                     const float* __restrict__ views,
+                                                     const float* __restrict__ points,
+                                                     const float* __restrict__ suns, int sstride,
+                                                     size_t n, float4* __restrict__ rad_out,
+                                                     float4* __restrict__ trans_out) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    size_t ci = cstride ? i : 0, si = sstride ? i : 0;
+    float3 cam = make_float3(cams[3 * ci], cams[3 * ci + 1], cams[3 * ci + 2]);
+    float3 sun = make_float3(suns[3 * si], suns[3 * si + 1], suns[3 * si + 2]);
+    float3 view = make_float3(views[3 * i], views[3 * i + 1], views[3 * i + 2]);
+    float3 rad, tr;
+    if (points) {
+        float3 pt = make_float3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
+        sky_radiance_to_point(P, T, S, cam, view, pt, sun, rad, tr);
+    } else {
+        sky_radiance(P, T, S, cam, view, sun, rad, tr);
+    }
+    rad_out[i] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    if (trans_out) trans_out[i] = make_float4(tr.x, tr.y, tr.z, 1.0f);
+}
+
+// Counter-based direction generator: SplitMix64 over (seed, frame, pixel,
+// attempt), Marsaglia's (1972) rejection method for a uniform point on the
+// sphere.  Only +, *, sqrt (IEEE) are used, so the host copy in
+// fb_ray_direction() produces identical bits.
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__host__ __device__ inline void ray_direction(uint64_t seed, uint32_t frame, uint32_t pixel,
+                                              float out[3]) {
+    uint64_t key = splitmix64(seed ^ (uint64_t(frame) << 32 | pixel));
+    for (uint32_t attempt = 0; attempt < 64; ++attempt) {
+        uint64_t h = splitmix64(key + attempt);
+        float v1 = float(uint32_t(h >> 40)) * (2.0f / 16777216.0f) - 1.0f;
+        float v2 = float(uint32_t((h >> 16) & 0xffffffu)) * (2.0f / 16777216.0f) - 1.0f;
+        float s = v1 * v1 + v2 * v2;
+        if (s < 1.0f && s > 0.0f) {
+            float q = 2.0f * sqrtf(1.0f - s);
+            out[0] = v1 * q;
+            out[1] = v2 * q;
+            out[2] = 1.0f - 2.0f * s;
+            return;
+        }
+    }
+    out[0] = 0.0f;
+    out[1] = 0.0f;
+    out[2] = 1.0f;
+}
+
+__global__ __launch_bounds__(256) void k_render_frames(DevParams P, const float4* __restrict__ T,
+                                                       const float4* __restrict__ S,
+                                                       const float* __restrict__ altitudes,
+                                                       const float* __restrict__ zeniths,
+                                                       uint32_t pixels, uint64_t seed,
+                                                       uint32_t frame0,
+                                                       float4* __restrict__ rad_out,
+                                                       float4* __restrict__ trans_out) {
+    uint32_t f = blockIdx.y;
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    float alt = altitudes[f], zs = zeniths[f];
+    float3 cam = make_float3(0.0f, 0.0f, P.bottom + alt);
+    float3 sun = make_float3(sinf(zs), 0.0f, cosf(zs));
+    float d[3];
+    ray_direction(seed, frame0 + f, p, d);
+    float3 rad, tr;
+    sky_radiance(P, T, S, cam, make_float3(d[0], d[1], d[2]), sun, rad, tr);
+    size_t o = size_t(f) * pixels + p;
+    rad_out[o] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    if (trans_out) trans_out[o] = make_float4(tr.x, tr.y, tr.z, 1.0f);
+}
+
+// render_lighting.h:10-28
+__global__ __launch_bounds__(256) void k_sun_sky_irradiance(
+    DevParams P, const float4* __restrict__ T, const float4* __restrict__ E,
+    const float* __restrict__ points, const float* __restrict__ normals,
+    const float* __restrict__ suns, int sstride, size_t n, float4* __restrict__ direct_out,
+    float4* __restrict__ sky_out) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    size_t si = sstride ? i : 0;
+    float3 pt = make_float3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
+    float3 nr = make_float3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+    float3 sun = make_float3(suns[3 * si], suns[3 * si + 1], suns[3 * si + 2]);
+    float r = sqrtf(dot3(pt, pt));
+    float mu_s = dot3(pt, sun) / r;
+    float3 e = get_irradiance<float>(P, E, r, mu_s);
+    float f = (1.0f + dot3(nr, pt) / r) * 0.5f;
+    sky_out[i] = make_float4(e.x * f, e.y * f, e.z * f, 0.0f);
+    float3 ts = transmittance_to_sun<float>(P, T, r, mu_s);
+    float ns = fmaxf(dot3(nr, sun), 0.0f);
+    direct_out[i] = make_float4(P.solar[0] * ts.x * ns, P.solar[1] * ts.y * ns,
+                                P.solar[2] * ts.z * ns, 0.0f);
+}
+
+}  // namespace fbk
+
+using namespace fbrt;
+
+static fb_status check_atm(fb_builder* b, const fb_atmosphere* a) {
+    if (!b || !a) {
+        set_error("NULL builder or atmosphere");
+        return FB_EINVAL;
+    }
+    if (a->builder != b) {
+        set_error("atmosphere belongs to another builder");
+        return FB_EINVAL;
+    }
+    return FB_OK;
+}
+
+// Order the render after the precompute and remember it as the
+// atmosphere's last use (destroy waits for it).
+static fb_status begin_use(fb_atmosphere* a, hipStream_t s) {
+    FB_HIP_TRY(hipSetDevice(a->builder->device));
+    FB_HIP_TRY(hipStreamWaitEvent(s, a->ready, 0));
+    return FB_OK;
+}
+static fb_status end_use(fb_atmosphere* a, hipStream_t s) {
+    FB_HIP_TRY(hipGetLastError());
+    FB_HIP_TRY(hipEventRecord(a->last_use, s));
+    a->used = true;
+    return FB_OK;
+}
+
+extern "C" {
+
+fb_status fb_render_sky(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,
+                        const fb_draw_parameters* draw, const float* depth, uint32_t width,
+                        uint32_t height, float* color, float* transmittance, uint32_t flags) {
+    fb_status st = check_atm(b, ac);
+    if (st != FB_OK) return st;
+    if (!draw || !color || width == 0 || height == 0) {
+        set_error("invalid render arguments");
+        return FB_EINVAL;
+    }
+    fb_atmosphere* a = const_cast<fb_atmosphere*>(ac);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = begin_use(a, s)) != FB_OK) return st;
+    fbk::DrawArgs D;
+    memcpy(D.ivp, draw->inverse_viewproj, sizeof(D.ivp));
+    memcpy(D.cam, draw->camera_position, sizeof(D.cam));
+    memcpy(D.sun, draw->sun_direction, sizeof(D.sun));
+    dim3 grid((width + 15) / 16, (height + 15) / 16);
+    fbk::k_render_sky<<<grid, 256, 0, s>>>(a->dev, a->T.p, a->S.p, D, depth, int(width),
+                                            int(height), reinterpret_cast<float4*>(color),
+                                            reinterpret_cast<float4*>(transmittance),
+                                            int(flags & FB_RENDER_COMPOSITE));
+    return end_use(a, s);
+}
+
+fb_status fb_render_sky_rays(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,
+                             const float* cameras, uint32_t camera_stride, const float* views,
+                             const float* points, const float* suns, uint32_t sun_stride,
+                             size_t n, float* radiance, float* transmittance) {
+    fb_status st = check_atm(b, ac);
+    if (st != FB_OK) return st;
+    if (!cameras || !views || !suns || !radiance || camera_stride > 1 || sun_stride > 1) {
+        set_error("invalid ray arguments");
+        return FB_EINVAL;
+    }
+    if (n == 0) return FB_OK;
+    fb_atmosphere* a = const_cast<fb_atmosphere*>(ac);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = begin_use(a, s)) != FB_OK) return st;
+    fbk::k_render_rays<<<unsigned((n + 255) / 256), 256, 0, s>>>(
+        a->dev, a->T.p, a->S.p, cameras, int(camera_stride), views, points, suns, int(sun_stride),
+        n, reinterpret_cast<float4*>(radiance), reinterpret_cast<float4*>(transmittance));
+    return end_use(a, s);
+}
+
+fb_status fb_render_sky_frames(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,
+                               const float* altitudes_km, const float* sun_zenith_rad,
+                               uint32_t n_frames, uint32_t pixels, uint64_t seed,
+                               float* radiance, float* transmittance) {
+    fb_status st = check_atm(b, ac);
+    if (st != FB_OK) return st;
+    if (!altitudes_km || !sun_zenith_rad || !radiance || n_frames > 65535) {
+        set_error("invalid frame arguments");
+        return FB_EINVAL;
+    }
+    if (n_frames == 0 || pixels == 0) return FB_OK;
+    fb_atmosphere* a = const_cast<fb_atmosphere*>(ac);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = begin_use(a, s)) != FB_OK) return st;
+    dim3 grid((pixels + 255) / 256, n_frames);
+    fbk::k_render_frames<<<grid, 256, 0, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
+                                               sun_zenith_rad, pixels, seed, 0u,
+                                               reinterpret_cast<float4*>(radiance),
+                                               reinterpret_cast<float4*>(transmittance));
+    return end_use(a, s);
+}
+
+void fb_ray_direction(uint64_t seed, uint32_t frame, uint32_t pixel, float out[3]) {
+    fbk::ray_direction(seed, frame, pixel, out);
+}
+
+fb_status fb_sun_and_sky_irradiance(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,
+                                    const float* points, const float* normals, const float* suns,
+                                    uint32_t sun_stride, size_t n, float* sun_irradiance,
+                                    float* sky_irradiance) {
+    fb_status st = check_atm(b, ac);
+    if (st != FB_OK) return st;
+    if (!points || !normals || !suns || !sun_irradiance || !sky_irradiance || sun_stride > 1) {
+        set_error("invalid lighting arguments");
+        return FB_EINVAL;
+    }
+    if (n == 0) return FB_OK;
+    fb_atmosphere* a = const_cast<fb_atmosphere*>(ac);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = begin_use(a, s)) != FB_OK) return st;
+    fbk::k_sun_sky_irradiance<<<unsigned((n + 255) / 256), 256, 0, s>>>(
+        a->dev, a->T.p, a->E.p, points, normals, suns, int(sun_stride), n,
+        reinterpret_cast<float4*>(sun_irradiance), reinterpret_cast<float4*>(sky_irradiance));
+    return end_use(a, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// runtime.hpp — internal host-side objects behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fuzzyblue_hip.h"
+#include "atmo_device.hpp"
+#include "kernels.hpp"
+
+namespace fbrt {
+
+// Thread-local last error (fb_last_error).
+void set_error(const char* fmt, ...);
+fb_status hip_fail(hipError_t e, const char* what);
+
+#define FB_HIP_TRY(expr)                                 \
+    do {                                                 \
+        hipError_t _e = (expr);                          \
+        if (_e != hipSuccess) return fbrt::hip_fail(_e, #expr); \
+    } while (0)
+
+// A caching device allocator.  Blocks are returned to a per-size free list
+// and reused; a block is handed back only by owners that have already
+// waited for the GPU work using it, so reuse never races.
+class Pool {
+  public:
+    explicit Pool(int device) : device_(device) {}
+    ~Pool();
+    fb_status alloc(size_t bytes, void** out);
+    void release(void* p, size_t bytes);
+    void trim();
+    size_t in_use() const { return in_use_; }
+    size_t cached() const { return cached_; }
+
+  private:
+    int device_;
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    size_t in_use_ = 0, cached_ = 0;
+};
+
+struct Buf {
+    float4* p = nullptr;
+    size_t bytes = 0;
+};
+
+fbk::DevParams make_dev_params(const fb_parameters& p);
+
+struct Sizes {
+    size_t nt, ni, ns;  // texel counts: transmittance, irradiance, scattering
+};
+Sizes sizes_of(const fb_parameters& p);
+
+}  // namespace fbrt
+
+// Opaque C-ABI objects.
+struct fb_builder {
+    int device = 0;
+    hipDeviceProp_t prop{};
+    std::unique_ptr<fbrt::Pool> pool;
+    hipStream_t capture_stream = nullptr;  // private stream for graph capture
+    std::mutex mu;
+};
+
+struct fb_atmosphere {
+    fb_builder* builder = nullptr;
+    fb_parameters params{};
+    fbk::DevParams dev{};
+    fbrt::Buf T, E, S;
+    hipEvent_t ready = nullptr;     // precompute complete
+    hipEvent_t last_use = nullptr;  // last render enqueued against it
+    bool used = false;
+};
+
+struct fb_pending_atmosphere {
+    fb_builder* builder = nullptr;
+    fb_atmosphere* atm = nullptr;
+    fbrt::Buf dE, dR, dM, dMS, J;
+    hipEvent_t done = nullptr;
+    hipGraphExec_t graph = nullptr;
+    std::shared_ptr<void> batch;  // shared state of a batched build
+};
+
+namespace fbrt {
+// Enqueue the whole pass schedule of one atmosphere on a stream.
+fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStream_t s);
+fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_a,
+                     fb_pending_atmosphere** out_pa);
+void free_pending_scratch(fb_pending_atmosphere* pa);
+void free_atmosphere_buffers(fb_atmosphere* a);
+}  // namespace fbrt

@@ -1,0 +1,117 @@
+"""C-ABI surface on the CPU: the library loads, exports every declared
+symbol, and the host-only entry points (defaults, extents, validation,
+footprint) behave like the reference's Parameters (src/precompute.rs)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from fuzzyblue_amd import Parameters, abi
+from fuzzyblue_amd.lib import check, declared_functions, load, FuzzyblueError
+
+
+def test_library_exports_every_declared_symbol():
+    lib = load()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_abi_version():
+    assert load().fb_abi_version() == abi.FB_ABI_VERSION
+
+
+def test_defaults_match_reference_default():
+    """fb_parameters_default == Parameters::default (precompute.rs:849-935)."""
+    c = abi.fb_parameters()
+    load().fb_parameters_default(C.byref(c))
+    assert bytes(c) == bytes(Parameters().to_c())
+    assert c.order == 4 and c.scattering_nu_size == 8 and c.bottom_radius == np.float32(6360.0)
+    assert c.mie_density.layers[1].exp_scale == np.float32(-0.833333)
+    assert c.absorption_density.layers[0].width == np.float32(25.0)
+    assert c.mu_s_min == np.float32(-0.207912)
+
+
+def test_extents():
+    """Parameters::*_extent (precompute.rs:771-793)."""
+    p = Parameters()
+    c = p.to_c()
+    t = (C.c_uint32 * 2)()
+    i = (C.c_uint32 * 2)()
+    s = (C.c_uint32 * 3)()
+    load().fb_parameters_extents(C.byref(c), t, i, s)
+    assert tuple(t) == p.transmittance_extent() == (256, 64)
+    assert tuple(i) == p.irradiance_extent() == (64, 16)
+    assert tuple(s) == p.scattering_extent() == (256, 128, 32)
+
+
+def test_footprint():
+    c = Parameters().to_c()
+    res = C.c_size_t()
+    scr = C.c_size_t()
+    tot = load().fb_parameters_footprint(C.byref(c), C.byref(res), C.byref(scr))
+    ns = 256 * 128 * 32
+    assert res.value == (256 * 64 + 64 * 16 + ns) * 16
+    assert scr.value == (64 * 16 + 4 * ns) * 16
+    assert tot == res.value + scr.value
+
+
+@pytest.mark.parametrize(
+    "field,value",
+    [
+        ("scattering_nu_size", 1),
+        ("scattering_mu_size", 2),
+        ("transmittance_mu_size", 0),
+        ("irradiance_r_size", 1),
+        ("order", 0),
+        ("top_radius", 6000.0),
+        ("bottom_radius", -1.0),
+        ("mie_phase_function_g", 1.0),
+        ("mu_s_min", 0.1),
+        ("sun_angular_radius", 0.2),
+        ("flags", 0x80),
+        ("scattering_nu_size", 1000),
+    ],
+)
+def test_validation_rejects(field, value):
+    p = Parameters().replace(**{field: value})
+    st = load().fb_parameters_validate(C.byref(p.to_c()))
+    assert st == abi.FB_EINVAL
+    assert load().fb_last_error()
+
+
+def test_validation_accepts_defaults_and_smoke_sizes():
+    from fuzzyblue_amd.params import SMOKE_SIZES, RESOLUTIONS
+
+    for p in [Parameters(), Parameters().replace(**SMOKE_SIZES)] + [
+        Parameters().replace(**r) for r in RESOLUTIONS.values()
+    ]:
+        check(load().fb_parameters_validate(C.byref(p.to_c())))
+
+
+def test_negative_coefficient_rejected():
+    p = Parameters()
+    p.rayleigh_scattering = [-1.0, 0.0, 0.0]
+    assert load().fb_parameters_validate(C.byref(p.to_c())) == abi.FB_EINVAL
+
+
+def test_status_strings():
+    lib = load()
+    for st in abi.STATUS_NAMES:
+        assert lib.fb_status_string(st).startswith(abi.STATUS_NAMES[st].encode())
+
+
+def test_params_roundtrip():
+    p = Parameters().replace(order=7, mie_phase_function_g=0.7)
+    q = Parameters.from_c(p.to_c())
+    assert bytes(q.to_c()) == bytes(p.to_c())
+
+
+def test_ray_direction_unit_and_deterministic():
+    from fuzzyblue_amd import ray_direction
+
+    for i in range(50):
+        d = ray_direction(2, 3, i)
+        assert abs(float(np.linalg.norm(d.astype(np.float64))) - 1.0) < 1e-6
+        assert np.array_equal(d, ray_direction(2, 3, i))
