@@ -17,7 +17,7 @@ constexpr int kIndTheta = 16;
 constexpr int kIndPhi = 64;
 
 // Size limits the kernels' LDS layouts accept (checked by the host).
-constexpr int kMaxNu = 64;
+constexpr int kMaxNu = 32;
 constexpr int kMaxIrrMuS = 1024;
 
 hipError_t upload_tables();

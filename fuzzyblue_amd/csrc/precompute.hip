@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_scattering_density(
 // and the wave reduces.  As in the density pass, the lookups of one theta
 // row share (u_r, u_mu, u_mu_s), so each wave first builds its 16 nu
 // profiles in LDS.
-constexpr int kIndWaves = 4;
+constexpr int kIndWaves = 4;  // waves per block (fewer when LDS would exceed 64 KiB)
 
 template <typename G>
 __global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
@@ -415,8 +415,9 @@ __global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
     float4* __restrict__ E) {
     // Dynamic LDS: per wave [ntex][16][NU+2] profile slots.
     extern __shared__ float4 s_dyn[];
+    const int waves = blockDim.x / 64;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int texel = blockIdx.x * kIndWaves + wave;
+    const int texel = blockIdx.x * waves + wave;
     const int n = P.i_mus * P.i_r;
     const bool active = texel < n;
     const int ti = active ? texel % P.i_mus : 0, tj = active ? texel / P.i_mus : 0;
@@ -577,13 +578,16 @@ hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float
                                       hipStream_t s) {
     size_t n = size_t(P.i_mus) * P.i_r;
     const int ntex = order == 1 ? 2 : 1;
-    size_t lds = sizeof(float4) * size_t(kIndWaves) * ntex * kIndTheta * (P.s_nu + 2);
+    size_t per_wave = sizeof(float4) * size_t(ntex) * kIndTheta * (P.s_nu + 2);
+    int waves = int(65536 / per_wave);
+    waves = waves < 1 ? 1 : (waves > kIndWaves ? kIndWaves : waves);
+    size_t lds = per_wave * waves;
     if (P.flags & 1)
         k_indirect_irradiance<double>
-            <<<cdiv(n, kIndWaves), 64 * kIndWaves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
+            <<<cdiv(n, waves), 64 * waves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
     else
         k_indirect_irradiance<float>
-            <<<cdiv(n, kIndWaves), 64 * kIndWaves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
+            <<<cdiv(n, waves), 64 * waves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
     return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ fb_status hip_fail(hipError_t e, const char* what);
 class Pool {
   public:
     explicit Pool(int device) : device_(device) {}
+    int device() const { return device_; }
     ~Pool();
     fb_status alloc(size_t bytes, void** out);
     void release(void* p, size_t bytes);
