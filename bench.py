@@ -1,0 +1,313 @@
+#!/usr/bin/env python3
+"""Benchmark: precompute LUT sets/s (+ p50/p99 latency) at <= 1e-3 relative
+error vs the FP64 oracle, on one MI355X.
+
+Workload (one "step"): the whole precompute of one default Earth atmosphere
+(Parameters::default(), src/precompute.rs:849-935: transmittance 256x64,
+irradiance 64x16, scattering 32x128x32x8, order 4) through the plain
+single-atmosphere HIP pipeline in precision mode, re-launched into the same
+resident buffers (the reference's bench likewise re-submits one recorded
+command buffer, benches/precompute.rs:138-148).  Inputs (the parameters) are
+resident; no host transfer is inside the timed region.
+
+Prints ONE JSON line.  Extra fields: latency percentiles of individual
+build requests, FP32-mode throughput, sky Mpixel/s (64 frames of
+1920x1080 rays, camera altitude U[0,10] km, sun zenith U[0,108] deg), the
+roofline of the dominant kernel (scattering density) and a CPU baseline
+(the FP32 restatement of the reference passes on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from fuzzyblue_amd import (  # noqa: E402
+    Atmosphere,
+    Builder,
+    DeviceBuffer,
+    Parameters,
+    Renderer,
+    Stream,
+    abi,
+    costmodel,
+    synchronize,
+)
+from fuzzyblue_amd.lib import check, load  # noqa: E402
+
+METRIC = "LUT-sets/s + p50/p99 latency at ≤1e-3 rel err vs FP64, 1×MI355X"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class Timer:
+    def __init__(self):
+        import ctypes as C
+
+        self.C = C
+        self.a = C.c_void_p()
+        self.b = C.c_void_p()
+        check(load().fb_event_create(C.byref(self.a)))
+        check(load().fb_event_create(C.byref(self.b)))
+
+    def start(self, stream):
+        check(load().fb_event_record(self.a, stream.handle if stream else None))
+
+    def stop(self, stream):
+        check(load().fb_event_record(self.b, stream.handle if stream else None))
+
+    def ms(self):
+        v = self.C.c_float()
+        check(load().fb_event_elapsed_ms(self.a, self.b, self.C.byref(v)))
+        return v.value
+
+
+def measure_throughput(b, stream, p, steps, warmup):
+    pa = Atmosphere.build(b, stream, p)
+    pa.wait()
+    for _ in range(warmup):
+        pa.relaunch(stream)
+    synchronize()
+    t = Timer()
+    t.start(stream)
+    for _ in range(steps):
+        pa.relaunch(stream)
+    t.stop(stream)
+    synchronize()
+    ms = t.ms()
+    return pa, ms
+
+
+def measure_latency(b, stream, p, n):
+    """Submit -> complete wall time of individual build requests (allocation
+    from the builder's cached pool included)."""
+    lat = []
+    for _ in range(n):
+        t0 = time.perf_counter()
+        pa = Atmosphere.build(b, stream, p)
+        pa.wait()
+        lat.append((time.perf_counter() - t0) * 1e3)
+        pa.close()
+    lat = np.array(lat)
+    return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def max_err_vs_fixture(atm):
+    path = os.path.join(ROOT, "tests", "golden", "earth_default_fp64.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+
+    def rel(x, y):
+        x = x.astype(np.float64)
+        y = y.astype(np.float64)
+        return float((np.abs(x - y) / np.maximum(np.abs(y), 1e-6 * np.abs(y).max())).max())
+
+    T = atm.download("transmittance")[..., :3]
+    E = atm.download("irradiance")[..., :3]
+    S = atm.download("scattering").reshape(-1, 4)[g["S_idx"]]
+    return {"transmittance": rel(T, g["T"]), "irradiance": rel(E, g["E"]),
+            "scattering": rel(S[:, :3], g["S"][:, :3])}
+
+
+def measure_render(b, stream, atm, frames=64, w=1920, h=1080, reps=3):
+    rng = np.random.default_rng(2)
+    alt = rng.uniform(0.0, 10.0, frames).astype(np.float32)
+    zen = np.radians(rng.uniform(0.0, 108.0, frames)).astype(np.float32)
+    d_alt = DeviceBuffer.from_array(alt)
+    d_zen = DeviceBuffer.from_array(zen)
+    px = w * h
+    rad = DeviceBuffer(frames * px * 16)
+    tr = DeviceBuffer(frames * px * 16)
+    r = Renderer(b)
+    r.frames(stream, atm, d_alt, d_zen, frames, px, 2, rad, tr)
+    synchronize()
+    t = Timer()
+    t.start(stream)
+    for _ in range(reps):
+        r.frames(stream, atm, d_alt, d_zen, frames, px, 2, rad, tr)
+    t.stop(stream)
+    ms = t.ms() / reps
+    for buf in (rad, tr, d_alt, d_zen):
+        buf.close()
+    return frames * px / (ms * 1e-3) / 1e6, ms
+
+
+def cpu_baseline(p):
+    """FP32 restatement of the reference passes on the host cores, timed on a
+    bounded sample: full transmittance / direct / indirect passes, and texel
+    subsets of single scattering, density and multiple scattering scaled to
+    the full LUT."""
+    from oracle import Oracle
+
+    o = Oracle("f32")
+    nt, ni, ns = costmodel.texel_counts(p)
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    T = o.pass_transmittance(p)
+    t_T = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dE = o.pass_direct_irradiance(p, T)
+    t_D = time.perf_counter() - t0
+    n_ss = 65536
+    idx = rng.choice(ns, n_ss, replace=False)
+    t0 = time.perf_counter()
+    o.single_texels(p, T, idx)
+    t_S = (time.perf_counter() - t0) * ns / n_ss
+    # Inputs for the later passes: the restatement's own single scattering.
+    dR, dM, S = o.pass_single_scattering(p, T)
+    zeros = np.zeros_like(dR)
+    n_j = 8192
+    jdx = rng.choice(ns, n_j, replace=False)
+    t0 = time.perf_counter()
+    o.density_texels(p, 2, T, dR, dM, zeros, dE, jdx)
+    t_J2 = (time.perf_counter() - t0) * ns / n_j
+    t0 = time.perf_counter()
+    o.density_texels(p, 3, T, dR, dM, dR, dE, jdx)
+    t_J3 = (time.perf_counter() - t0) * ns / n_j
+    t0 = time.perf_counter()
+    o.pass_indirect(p, 1, dR, dM, zeros, np.zeros_like(dE))
+    t_I = time.perf_counter() - t0
+    n_m = 65536
+    mdx = rng.choice(ns, n_m, replace=False)
+    t0 = time.perf_counter()
+    o.multiple_texels(p, T, dR, mdx)
+    t_M = (time.perf_counter() - t0) * ns / n_m
+    n_orders = p.order - 1
+    t_set = t_T + t_D + t_S + t_J2 + (n_orders - 1) * t_J3 + n_orders * (t_I + t_M)
+    cores = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return {
+        "value": 1.0 / t_set,
+        "unit": "LUT-sets/s",
+        "cores": min(cores, threads),
+        "kind": "port",
+        "sample": (
+            "FP32 restatement (oracle/atmo_oracle.c, OpenMP) of the reference passes for the "
+            "default Earth LUT set, order 4: full transmittance/direct/indirect passes; "
+            f"{n_ss} single-scattering, {n_j} density (orders 2 and 3) and {n_m} "
+            "multiple-scattering texels, scaled to the full 1,048,576-texel LUT"
+        ),
+        "seconds_per_set": t_set,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--latency-requests", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-render", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: the workload is one atmosphere's LUT set on one "
+            "MI355X; multi-GPU runs are not part of this build (atmospheres shard with no "
+            "exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    b = Builder(0)
+    stream = Stream()
+    p = Parameters().replace(flags=abi.FB_FLAG_PRECISE)
+    log("[bench] precision-mode throughput")
+    pa, ms = measure_throughput(b, stream, p, args.steps, args.warmup)
+    value = args.steps / (ms * 1e-3)
+    ms_per_step = ms / args.steps
+    log(f"[bench] {value:.1f} LUT-sets/s ({ms_per_step:.3f} ms/set)")
+
+    # Per-launch profile of the same schedule (HIP events on the launch stream).
+    prof = pa.profile(stream)
+    passes = {}
+    for name, order, t in prof:
+        passes.setdefault(name, []).append(t)
+    dens = [(order, t) for name, order, t in prof if name == "scattering_density"]
+    dens_flop = sum(costmodel.pass_work(p, "scattering_density", o).flop for o, _ in dens)
+    dens_ms = sum(t for _, t in dens)
+    achieved = dens_flop / (dens_ms * 1e-3) / 1e12
+    dens_bytes = sum(costmodel.pass_work(p, "scattering_density", o).bytes for o, _ in dens)
+
+    err = max_err_vs_fixture(pa.atmosphere())
+    log(f"[bench] max rel err vs FP64 oracle: {err}")
+
+    log("[bench] latency")
+    p50, p99 = measure_latency(b, stream, p, args.latency_requests)
+
+    log("[bench] fp32-mode throughput")
+    pa32, ms32 = measure_throughput(b, stream, p.replace(flags=0), args.steps, args.warmup)
+    pa32.close()
+
+    sky = None
+    if not args.no_render:
+        log("[bench] render")
+        mpx, rms = measure_render(b, stream, pa.atmosphere())
+        sky = {"mpixel_s": mpx, "ms_per_64_frames": rms, "frames": 64, "width": 1920,
+               "height": 1080}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        cpu = cpu_baseline(Parameters())
+
+    work = costmodel.lut_set_work(p)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "LUT-sets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain single-atmosphere precompute of the default Earth atmosphere "
+                        "(order 4; LUTs 256x64, 64x16, 32x128x32x8), precision mode (FP64 "
+                        "geometry, FP32 radiance)",
+            "order": p.order,
+            "transmittance": list(p.transmittance_extent()),
+            "irradiance": list(p.irradiance_extent()),
+            "scattering_r_mu_mus_nu": [p.scattering_r_size, p.scattering_mu_size,
+                                       p.scattering_mu_s_size, p.scattering_nu_size],
+            "precision": "precise",
+        },
+        "latency_ms": {"p50": p50, "p99": p99, "requests": args.latency_requests},
+        "max_rel_err_vs_fp64": err,
+        "fp32_mode_lut_sets_s": args.steps / (ms32 * 1e-3),
+        "sky": sky,
+        "roofline": {
+            "bound": "valu",
+            "kernel": "scattering_density",
+            "achieved": achieved,
+            "peak": costmodel.PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / costmodel.PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": dens_bytes / max(len(dens), 1),
+            "avg_launch_ms": dens_ms / max(len(dens), 1),
+        },
+        "lut_set_roofline_ms": work.roof_seconds() * 1e3,
+        "passes_ms": {k: float(np.sum(v)) for k, v in passes.items()},
+        "cpu_baseline": cpu,
+    }
+    pa.close()
+    stream.close()
+    b.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

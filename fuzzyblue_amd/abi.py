@@ -96,6 +96,21 @@ class fb_parameters(C.Structure):
     ]
 
 
+FB_PASS_NAMES = {
+    0: "transmittance",
+    1: "direct_irradiance",
+    2: "single_scattering",
+    3: "clear_irradiance",
+    4: "scattering_density",
+    5: "indirect_irradiance",
+    6: "multiple_scattering",
+}
+
+
+class fb_pass_timing(C.Structure):
+    _fields_ = [("pass_", C.c_uint32), ("order", C.c_uint32), ("ms", C.c_float)]
+
+
 class fb_lut_view(C.Structure):
     _fields_ = [
         ("device_ptr", C.c_void_p),

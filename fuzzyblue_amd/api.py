@@ -225,6 +225,16 @@ class PendingAtmosphere:
     def relaunch(self, stream=None):
         check(load().fb_pending_relaunch(self.handle, _stream_handle(stream)))
 
+    def profile(self, stream=None):
+        """Per-launch GPU times of one re-run: list of (pass, order, ms)."""
+        cap = 8 + 3 * 64
+        arr = (abi.fb_pass_timing * cap)()
+        n = C.c_size_t()
+        check(load().fb_pending_profile(self.handle, _stream_handle(stream), arr, cap,
+                                        C.byref(n)))
+        return [(abi.FB_PASS_NAMES[arr[i].pass_], arr[i].order, arr[i].ms)
+                for i in range(min(n.value, cap))]
+
     def scratch(self, which) -> abi.fb_lut_view:
         v = abi.fb_lut_view()
         check(load().fb_pending_lut(self.handle, _lut_id(which), C.byref(v)))

@@ -18,7 +18,8 @@ constexpr int kIndPhi = 64;
 
 // Size limits the kernels' LDS layouts accept (checked by the host).
 constexpr int kMaxNu = 32;
-constexpr int kMaxIrrMuS = 1024;
+constexpr int kMaxIrrMuS = 256;
+constexpr int kMaxRowTexels = 1024;  // NU * MUS: one scattering row per block
 
 hipError_t upload_tables();
 

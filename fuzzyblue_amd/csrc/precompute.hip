@@ -168,53 +168,100 @@ __global__ __launch_bounds__(256) void k_direct_irradiance(DevParams P,
         make_float4(P.solar[0] * t.x * a, P.solar[1] * t.y * a, P.solar[2] * t.z * a, 0.0f);
 }
 
+// Trapezoid samples of the single and multiple scattering integrals:
+// SAMPLE_COUNT + 1 (single_scattering.comp:42, multiple_scattering.comp:21).
+constexpr int kMsSamples = 51;
+
 // ===================================================== 3. single scattering
-// single_scattering.comp:10-101.  One thread per texel, x fastest, so the
-// three LUT stores of a wave are contiguous.
+// single_scattering.comp:10-101.
+//
+// As for multiple scattering, the texels of one (r, mu) row share the view
+// ray: for each of the 51 samples the block computes once d_i, r_d, the
+// view transmittance T(r, mu, d_i), the densities at r_d and the
+// transmittance-LUT row taps and horizon terms of r_d.  Each texel then
+// evaluates only its sun transmittance T(r_d, mu_s_d) (one bilinear
+// lookup) and accumulates.
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void k_single_scattering(DevParams P,
                                                            const float4* __restrict__ T,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
                                                            float4* __restrict__ S) {
-    const unsigned W = unsigned(P.s_nu * P.s_mus);
-    const size_t n = size_t(W) * P.s_mu * P.s_r;
-    size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    unsigned x = unsigned(idx % W), y = unsigned((idx / W) % P.s_mu),
-             z = unsigned(idx / (size_t(W) * P.s_mu));
-    TexelCoord<G> c = scattering_texel_coord<G>(P, x, y, z);
-    const int SAMPLE_COUNT = 50;
-    G dx = distance_to_nearest<G>(P, c.r, c.mu, c.ground) / G(SAMPLE_COUNT);
-    Sum<COMP> rs[3], ms[3];
-    for (int i = 0; i <= SAMPLE_COUNT; ++i) {
+    __shared__ float4 s_tv[kMsSamples];   // T(r,mu,d_i) * weight (rgb), d_i
+    __shared__ float4 s_geo[kMsSamples];  // r_d, rho_rayleigh(r_d), rho_mie(r_d), sin_theta_h
+    __shared__ float4 s_lut[kMsSamples];  // cos_theta_h, d_min, d_max - d_min, T-LUT row weight
+    __shared__ int2 s_trow[kMsSamples];   // T-LUT rows of r_d
+    const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
+    const int W = NU * MUS;
+    const int y = blockIdx.x, z = blockIdx.y;
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    TexelCoord<G> rc = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z));
+    const G r = rc.r, mu = rc.mu;
+    const bool ground = rc.ground;
+    const G dx = distance_to_nearest<G>(P, r, mu, ground) / G(kMsSamples - 1);
+    if (tid < kMsSamples) {
+        const int i = tid;
         G d_i = G(i) * dx;
-        G r_d = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * c.r * c.mu * d_i + c.r * c.r));
-        G mu_s_d = clamp_cosine<G>((c.r * c.mu_s + d_i * c.nu) / r_d);
-        float3 tv = get_transmittance<G>(P, T, c.r, c.mu, d_i, c.ground);
-        float3 ts = transmittance_to_sun<G>(P, T, r_d, mu_s_d);
+        G r_d = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r));
+        float3 tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
+        float w = (i == 0 || i == kMsSamples - 1) ? 0.5f : 1.0f;
         float alt = float(r_d - G(P.bottom));
-        float rd = profile_density(P.prof[0], alt);
-        float md = profile_density(P.prof[1], alt);
-        float w = (i == 0 || i == SAMPLE_COUNT) ? 0.5f : 1.0f;
-        float t0 = tv.x * ts.x, t1 = tv.y * ts.y, t2 = tv.z * ts.z;
-        rs[0].add(t0 * rd * w);
-        rs[1].add(t1 * rd * w);
-        rs[2].add(t2 * rd * w);
-        ms[0].add(t0 * md * w);
-        ms[1].add(t1 * md * w);
-        ms[2].add(t2 * md * w);
+        G sin_h = G(P.bottom) / r_d;
+        G cos_h = -sqrt(fmax(G(1) - sin_h * sin_h, G(0)));
+        // transmittance.h:7-24 terms that depend on r_d only.
+        G H = sqrt(G(P.top) * G(P.top) - G(P.bottom) * G(P.bottom));
+        G rho = safe_sqrt<G>(r_d * r_d - G(P.bottom) * G(P.bottom));
+        G d_min = G(P.top) - r_d, d_max = rho + H;
+        Taps tr = linear_taps<G>(tex_from_unit<G>(rho / H, P.t_r), P.t_r);
+        s_tv[i] = make_float4(tv.x * w, tv.y * w, tv.z * w, float(d_i));
+        s_geo[i] = make_float4(float(r_d), profile_density(P.prof[0], alt),
+                               profile_density(P.prof[1], alt), float(sin_h));
+        s_lut[i] = make_float4(float(cos_h), float(d_min), float(d_max - d_min), tr.w);
+        s_trow[i] = make_int2(tr.i0 * P.t_mu, tr.i1 * P.t_mu);
     }
-    float dxf = float(dx);
-    float3 ray = make_float3(rs[0].value() * dxf * P.solar[0] * P.rayleigh_scattering[0],
-                             rs[1].value() * dxf * P.solar[1] * P.rayleigh_scattering[1],
-                             rs[2].value() * dxf * P.solar[2] * P.rayleigh_scattering[2]);
-    float3 mie = make_float3(ms[0].value() * dxf * P.solar[0] * P.mie_scattering[0],
-                             ms[1].value() * dxf * P.solar[1] * P.mie_scattering[1],
-                             ms[2].value() * dxf * P.solar[2] * P.mie_scattering[2]);
-    dR[idx] = make_float4(ray.x, ray.y, ray.z, 0.0f);
-    dM[idx] = make_float4(mie.x, mie.y, mie.z, 0.0f);
-    S[idx] = make_float4(ray.x, ray.y, ray.z, mie.x);
+    __syncthreads();
+    const float rf = float(r);
+    const float alpha = P.sun_angular_radius;
+    const float top2 = P.top * P.top;
+    for (int x = tid; x < W; x += nt) {
+        TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
+        const float mu_s = float(c.mu_s), nu = float(c.nu);
+        Sum<COMP> rs[3], ms[3];
+        for (int i = 0; i < kMsSamples; ++i) {
+            const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i];
+            const int2 rows = s_trow[i];
+            const float r_d = gq.x, d_i = tv.w;
+            const float mu_s_d = clamp_cosine<float>((rf * mu_s + d_i * nu) / r_d);
+            // Sun transmittance T(r_d, mu_s_d) (transmittance.h:26-33, 63-74).
+            float disc = r_d * r_d * (mu_s_d * mu_s_d - 1.0f) + top2;
+            float d = clamp_distance<float>(-r_d * mu_s_d + safe_sqrt<float>(disc));
+            float x_mu = (d - lq.y) / lq.z;
+            Taps tx = linear_taps<float>(tex_from_unit<float>(x_mu, P.t_mu), P.t_mu);
+            float4 ts = lerp4(lerp4(ldg4(T + rows.x + tx.i0), ldg4(T + rows.x + tx.i1), tx.w),
+                              lerp4(ldg4(T + rows.y + tx.i0), ldg4(T + rows.y + tx.i1), tx.w),
+                              lq.w);
+            float sm = smoothstep_g<float>(-gq.w * alpha, gq.w * alpha, mu_s_d - lq.x);
+            float t0 = tv.x * (ts.x * sm), t1 = tv.y * (ts.y * sm), t2 = tv.z * (ts.z * sm);
+            rs[0].add(t0 * gq.y);
+            rs[1].add(t1 * gq.y);
+            rs[2].add(t2 * gq.y);
+            ms[0].add(t0 * gq.z);
+            ms[1].add(t1 * gq.z);
+            ms[2].add(t2 * gq.z);
+        }
+        const float dxf = float(dx);
+        float3 ray = make_float3(rs[0].value() * dxf * P.solar[0] * P.rayleigh_scattering[0],
+                                 rs[1].value() * dxf * P.solar[1] * P.rayleigh_scattering[1],
+                                 rs[2].value() * dxf * P.solar[2] * P.rayleigh_scattering[2]);
+        float3 mie = make_float3(ms[0].value() * dxf * P.solar[0] * P.mie_scattering[0],
+                                 ms[1].value() * dxf * P.solar[1] * P.mie_scattering[1],
+                                 ms[2].value() * dxf * P.solar[2] * P.mie_scattering[2]);
+        const size_t idx = (size_t(z) * MU + y) * W + x;
+        dR[idx] = make_float4(ray.x, ray.y, ray.z, 0.0f);
+        dM[idx] = make_float4(mie.x, mie.y, mie.z, 0.0f);
+        S[idx] = make_float4(ray.x, ray.y, ray.z, mie.x);
+    }
 }
 
 // ===================================================== 4. scattering density
@@ -250,83 +297,123 @@ __device__ __forceinline__ float4 trilinear_slice(const DevParams& P,
                  tz.w);
 }
 
-// Lerp of a nu profile (LDS, slots -1..NU stored at 0..NU+1) at nu.
-__device__ __forceinline__ float4 profile_lookup(const float4* prof, int nu_size, float nu) {
-    float u_nu = (nu + 1.0f) / 2.0f;
-    float tex_coord_x = u_nu * float(nu_size - 1);
-    float tex_x = floorf(tex_coord_x);
-    float lerp = tex_coord_x - tex_x;
-    tex_x = fminf(fmaxf(tex_x, -1.0f), float(nu_size - 1));
-    int e = int(tex_x) + 1;
-    float4 a = prof[e], b = prof[e + 1];
-    return make_float4(a.x * (1.0f - lerp) + b.x * lerp, a.y * (1.0f - lerp) + b.y * lerp,
-                       a.z * (1.0f - lerp) + b.z * lerp, a.w * (1.0f - lerp) + b.w * lerp);
+// A profile entry: the value at slot k (rgb) and the difference to slot
+// k+1, so a lerp is one fma per channel.  32-byte stride; a lookup reads
+// 24 bytes (one b128 and one b64 LDS load).
+struct ProfEntry {
+    float4 a;  // v.r, v.g, v.b, d.r
+    float2 b;  // d.g, d.b
+    float2 pad;
+};
+
+// Linear filtering at texel coordinate t over entries e = k+1, k in
+// [-1, kmax]: entry 0 holds the clamped left edge and entry kmax+1 the
+// clamped right edge, which reproduces CLAMP_TO_EDGE.
+__device__ __forceinline__ float3 prof_lerp(const ProfEntry* __restrict__ prof, float t,
+                                            int kmax) {
+    float k = floorf(t);
+    float f = t - k;
+    k = fminf(fmaxf(k, -1.0f), float(kmax));
+    const ProfEntry* e = prof + (int(k) + 1);
+    float4 a = e->a;
+    float2 b = e->b;
+    return make_float3(fmaf(a.w, f, a.x), fmaf(b.x, f, a.y), fmaf(b.y, f, a.z));
 }
 
-template <typename G>
+__device__ __forceinline__ ProfEntry make_entry(float4 v, float4 w) {
+    ProfEntry e;
+    e.a = make_float4(v.x, v.y, v.z, w.x - v.x);
+    e.b = make_float2(w.y - v.y, w.z - v.z);
+    e.pad = make_float2(0.0f, 0.0f);
+    return e;
+}
+
+inline size_t density_lds_bytes(int nu, int i_mus, bool single) {
+    const int ntex = single ? 2 : 1;
+    return sizeof(float4) * (size_t(ntex) * kDenTheta * (nu + 2) + i_mus) +
+           sizeof(ProfEntry) * (size_t(ntex) * kDenTheta * (nu + 1) + i_mus + 1);
+}
+
+// Per-sample body of the density quadrature for one pair of directions
+// (phi_m, phi_{31-m}): both share cos(phi), hence nu2 and the two phase
+// functions; their sin(phi) terms have opposite signs.
+template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256) void k_scattering_density(
-    DevParams P, int scattering_order, const float4* __restrict__ T,
-    const float4* __restrict__ dR, const float4* __restrict__ dM,
-    const float4* __restrict__ dMS, const float4* __restrict__ dE, float4* __restrict__ J) {
-    // Dynamic LDS: [ntex][16][NU+2] profile slots, then the I_MUS-texel
-    // delta_irradiance row at r = bottom.
+    DevParams P, const float4* __restrict__ T, const float4* __restrict__ dR,
+    const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
+    float4* __restrict__ J) {
     extern __shared__ float4 s_dyn[];
-    __shared__ float4 s_ground[kDenTheta];  // (T_g*albedo/pi).rgb, distance_to_ground
+    __shared__ float4 s_gv[kDenTheta];    // (T_g*albedo/pi).rgb, distance_to_ground
+    __shared__ float s_ginv[kDenTheta];   // 1/|r*zenith + omega_i*d_g|
     __shared__ int s_ground_mask;
 
-    const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
+    constexpr int NTEX = SINGLE ? 2 : 1;
+    const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu, NE = P.i_mus;
     const int W = NU * MUS;
+    const int nslots = NU + 2, nent = NU + 1;
     const int j = blockIdx.y;  // mu_s index within a nu slice
     const int z = blockIdx.z;  // r index
     const int tid = threadIdx.x;
-    const bool single = (scattering_order - 1) == 1;
-    const int nslots = NU + 2;
-    float4* s_prof = s_dyn;                                      // [tex][l][slot]
-    float4* s_erow = s_dyn + (single ? 2 : 1) * kDenTheta * nslots;
+    float4* s_raw = s_dyn;                                   // [NTEX][16][nslots]
+    float4* s_eraw = s_raw + NTEX * kDenTheta * nslots;      // [NE]
+    ProfEntry* s_prof = reinterpret_cast<ProfEntry*>(s_eraw + NE);  // [NTEX][16][nent]
+    ProfEntry* s_erow = s_prof + NTEX * kDenTheta * nent;    // [NE+1]
 
-    // Tile coordinates: r and mu_s shared by the tile (texel x = j, y = 0).
+    // Tile: r and mu_s shared by every texel (texel x = j, y = 0).
     TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
     const G r = tc.r;
-    const G u_r = scattering_u_r<G>(P, r);
     const G u_mu_s = scattering_u_mu_s<G>(P, tc.mu_s);
-    const Taps tz = linear_taps<G>(u_r, P.s_r);
+    const Taps tz = linear_taps<G>(scattering_u_r<G>(P, r), P.s_r);
 
-    // Profiles: 16 theta rows x (NU+2) slots x (2 textures at order 1).
-    const int nprof = kDenTheta * nslots * (single ? 2 : 1);
-    for (int e = tid; e < nprof; e += blockDim.x) {
+    // 1. Raw profiles: 16 theta rows x (NU+2) nu slots (-1..NU) per texture.
+    for (int e = tid; e < NTEX * kDenTheta * nslots; e += blockDim.x) {
         int tex_sel = e / (kDenTheta * nslots);
         int rem = e % (kDenTheta * nslots);
         int l = rem / nslots, slot = rem % nslots;
         G ct = den_cos<G>(l);
         bool g_l = ray_intersects_ground<G>(P, r, ct);
-        G u_mu = scattering_u_mu<G>(P, r, ct, g_l);
-        Taps ty = linear_taps<G>(u_mu, MU);
-        const float4* tex = single ? (tex_sel == 0 ? dR : dM) : dMS;
-        s_prof[(tex_sel * kDenTheta + l) * nslots + slot] =
-            trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
+        Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r, ct, g_l), MU);
+        const float4* tex = SINGLE ? (tex_sel == 0 ? dR : dM) : dMS;
+        s_raw[e] = trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
     }
-    // Ground terms per theta row (scattering_density.comp:45-60).
+    // delta_irradiance at r = bottom (irradiance.h:20-30 with x_r = 0).
+    {
+        Taps ty = linear_taps<G>(tex_from_unit<G>(G(0), P.i_r), P.i_r);
+        for (int x = tid; x < NE; x += blockDim.x)
+            s_eraw[x] = lerp4(ldg4(dE + ty.i0 * NE + x), ldg4(dE + ty.i1 * NE + x), ty.w);
+    }
+    // Ground terms per theta row (scattering_density.comp:45-60).  The
+    // ground normal's length does not depend on phi.
     if (tid < kDenTheta) {
         int l = tid;
         G ct = den_cos<G>(l);
         bool g_l = ray_intersects_ground<G>(P, r, ct);
         float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float ginv = 0.0f;
         if (g_l) {
             G dg = distance_to_bottom<G>(P, r, ct);
             float3 tg = get_transmittance<G>(P, T, r, ct, dg, true);
             gv = make_float4(tg.x * P.albedo[0] * (1.0f / kPi), tg.y * P.albedo[1] * (1.0f / kPi),
                              tg.z * P.albedo[2] * (1.0f / kPi), float(dg));
+            float4 o = c_den_dir[l * kDenPhi];
+            G st2 = G(o.x) * G(o.x) + G(o.y) * G(o.y);
+            G gz = r + ct * dg;
+            ginv = float(G(1) / sqrt(st2 * dg * dg + gz * gz));
         }
-        s_ground[l] = gv;
+        s_gv[l] = gv;
+        s_ginv[l] = ginv;
         unsigned long long b = __ballot(g_l);
         if (l == 0) s_ground_mask = int(b & 0xffffull);
     }
-    // delta_irradiance row at r = bottom (irradiance.h:20-30 with x_r = 0).
-    {
-        G uy = tex_from_unit<G>(G(0), P.i_r);
-        Taps ty = linear_taps<G>(uy, P.i_r);
-        for (int x = tid; x < P.i_mus; x += blockDim.x)
-            s_erow[x] = lerp4(ldg4(dE + ty.i0 * P.i_mus + x), ldg4(dE + ty.i1 * P.i_mus + x), ty.w);
+    __syncthreads();
+    // 2. (value, delta) entries.
+    for (int e = tid; e < NTEX * kDenTheta * nent; e += blockDim.x) {
+        int row = e / nent, k = e % nent;  // entry k holds slot k-1
+        s_prof[e] = make_entry(s_raw[row * nslots + k], s_raw[row * nslots + k + 1]);
+    }
+    for (int e = tid; e <= NE; e += blockDim.x) {
+        int i0 = min(max(e - 1, 0), NE - 1), i1 = min(e, NE - 1);
+        s_erow[e] = make_entry(s_eraw[i0], s_eraw[i1]);
     }
     __syncthreads();
 
@@ -342,7 +429,7 @@ __global__ __launch_bounds__(256) void k_scattering_density(
     G sdy = sqrt(fmax(G(1) - sdx * sdx - c.mu_s * c.mu_s, G(0)));
     const float omx = float(om_x), omz = float(c.mu);
     const float osx = float(sdx), osy = float(sdy), osz = float(c.mu_s);
-    const float rf = float(c.r);
+    const float rf = float(r);
     const float alt = float(c.r - G(P.bottom));
     const float rho_r = profile_density(P.prof[0], alt);
     const float rho_m = profile_density(P.prof[1], alt);
@@ -352,53 +439,77 @@ __global__ __launch_bounds__(256) void k_scattering_density(
                 cm2 = P.mie_scattering[2] * rho_m;
     const int gmask = s_ground_mask;
     const float g = P.g;
+    const float kR = 3.0f / (16.0f * kPi);
+    const float kM = 3.0f / (8.0f * kPi) * (1.0f - g * g) / (2.0f + g * g);
+    const float g1 = 1.0f + g * g, g2 = -2.0f * g;
+    const float hnu = 0.5f * float(NU - 1), hE = 0.5f * float(NE - 1);
+    const int kmax = NU - 1, emax = NE - 1;
 
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     for (int l = 0; l < kDenTheta; ++l) {
-        const float4* pr = s_prof + l * nslots;
-        const float4* pm = s_prof + (kDenTheta + l) * nslots;
+        const float4 d0 = c_den_dir[l * kDenPhi];
+        const float ct = d0.z, dom = d0.w;
+        const float C = ct * osz, omzc = ct * omz;
+        const float r0 = cr0 * dom, r1 = cr1 * dom, r2 = cr2 * dom;
+        const float m0 = cm0 * dom, m1 = cm1 * dom, m2 = cm2 * dom;
+        const ProfEntry* pr = s_prof + l * nent;
+        const ProfEntry* pm = s_prof + (kDenTheta + l) * nent;
         const bool ground_l = (gmask >> l) & 1;
-        const float4 gv = s_ground[l];
-        for (int m = 0; m < kDenPhi; ++m) {
-            const float4 oi = c_den_dir[l * kDenPhi + m];
-            float nu1 = osx * oi.x + osy * oi.y + osz * oi.z;
-            float i0, i1, i2;
-            if (single) {
-                float4 vr = profile_lookup(pr, NU, nu1);
-                float4 vm = profile_lookup(pm, NU, nu1);
-                float prr = phase_rayleigh(nu1), pmm = phase_mie(g, nu1);
-                i0 = vr.x * prr + vm.x * pmm;
-                i1 = vr.y * prr + vm.y * pmm;
-                i2 = vr.z * prr + vm.z * pmm;
-            } else {
-                float4 v = profile_lookup(pr, NU, nu1);
-                i0 = v.x;
-                i1 = v.y;
-                i2 = v.z;
+        float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float Ag = 0.0f, Bg = 0.0f, Cg = 0.0f;
+        if (ground_l) {
+            gv = s_gv[l];
+            float inv = s_ginv[l];
+            float dgi = gv.w * inv;
+            Ag = dgi * osx;
+            Bg = dgi * osy;
+            Cg = (rf + ct * gv.w) * inv * osz;
+        }
+#pragma unroll 2
+        for (int m = 0; m < kDenPhi / 2; ++m) {
+            const float4 o = c_den_dir[l * kDenPhi + m];
+            const float cx = o.x, sy = o.y;
+            // Shared: phase functions at nu2 = omega . omega_i.
+            const float nu2 = fmaf(cx, omx, omzc);
+            const float q = fmaf(nu2, nu2, 1.0f);
+            const float rs = rsqrtf(fmaf(g2, nu2, g1));
+            const float pR = kR * q, pM = kM * q * (rs * rs * rs);
+            const float s0 = fmaf(m0, pM, r0 * pR), s1 = fmaf(m1, pM, r1 * pR),
+                        s2 = fmaf(m2, pM, r2 * pR);
+            // The pair's incident radiance.
+            const float base = fmaf(cx, osx, C), sb = sy * osy;
+            const float nua = base + sb, nub = base - sb;
+            float3 La = prof_lerp(pr, fmaf(nua, hnu, hnu), kmax);
+            float3 Lb = prof_lerp(pr, fmaf(nub, hnu, hnu), kmax);
+            if (SINGLE) {
+                float3 Ma = prof_lerp(pm, fmaf(nua, hnu, hnu), kmax);
+                float3 Mb = prof_lerp(pm, fmaf(nub, hnu, hnu), kmax);
+                float qa = fmaf(nua, nua, 1.0f), qb = fmaf(nub, nub, 1.0f);
+                float ra = rsqrtf(fmaf(g2, nua, g1)), rb = rsqrtf(fmaf(g2, nub, g1));
+                float pRa = kR * qa, pMa = kM * qa * (ra * ra * ra);
+                float pRb = kR * qb, pMb = kM * qb * (rb * rb * rb);
+                La = make_float3(fmaf(Ma.x, pMa, La.x * pRa), fmaf(Ma.y, pMa, La.y * pRa),
+                                 fmaf(Ma.z, pMa, La.z * pRa));
+                Lb = make_float3(fmaf(Mb.x, pMb, Lb.x * pRb), fmaf(Mb.y, pMb, Lb.y * pRb),
+                                 fmaf(Mb.z, pMb, Lb.z * pRb));
             }
             if (ground_l) {
-                float gx = oi.x * gv.w, gy = oi.y * gv.w, gz = rf + oi.z * gv.w;
-                float len = sqrtf(gx * gx + gy * gy + gz * gz);
-                gx = gx / len;
-                gy = gy / len;
-                gz = gz / len;
-                float gmu = gx * osx + gy * osy + gz * osz;
-                float u = tex_from_unit<float>(gmu * 0.5f + 0.5f, P.i_mus);
-                Taps tx = linear_taps<float>(u, P.i_mus);
-                float4 e = lerp4(s_erow[tx.i0], s_erow[tx.i1], tx.w);
-                i0 += gv.x * e.x;
-                i1 += gv.y * e.y;
-                i2 += gv.z * e.z;
+                const float gb = fmaf(cx, Ag, Cg), gs = sy * Bg;
+                float3 Ea = prof_lerp(s_erow, fmaf(gb + gs, hE, hE), emax);
+                float3 Eb = prof_lerp(s_erow, fmaf(gb - gs, hE, hE), emax);
+                La = make_float3(fmaf(gv.x, Ea.x, La.x), fmaf(gv.y, Ea.y, La.y),
+                                 fmaf(gv.z, Ea.z, La.z));
+                Lb = make_float3(fmaf(gv.x, Eb.x, Lb.x), fmaf(gv.y, Eb.y, Lb.y),
+                                 fmaf(gv.z, Eb.z, Lb.z));
             }
-            float nu2 = omx * oi.x + 0.0f * oi.y + omz * oi.z;
-            float prr = phase_rayleigh(nu2), pmm = phase_mie(g, nu2);
-            a0 += i0 * (cr0 * prr + cm0 * pmm) * oi.w;
-            a1 += i1 * (cr1 * prr + cm1 * pmm) * oi.w;
-            a2 += i2 * (cr2 * prr + cm2 * pmm) * oi.w;
+            a0 = fmaf(La.x + Lb.x, s0, a0);
+            a1 = fmaf(La.y + Lb.y, s1, a1);
+            a2 = fmaf(La.z + Lb.z, s2, a2);
         }
     }
     J[(size_t(z) * MU + y) * W + x] = make_float4(a0, a1, a2, 0.0f);
 }
+
 
 // =================================================== 5. indirect irradiance
 // indirect_irradiance.comp:10-74.  One wavefront per irradiance texel: lane
@@ -408,68 +519,77 @@ __global__ __launch_bounds__(256) void k_scattering_density(
 // profiles in LDS.
 constexpr int kIndWaves = 4;  // waves per block (fewer when LDS would exceed 64 KiB)
 
-template <typename G>
+__host__ __device__ inline size_t indirect_lds_bytes_per_wave(int nu, bool single) {
+    const int ntex = single ? 2 : 1;
+    return sizeof(float4) * size_t(ntex) * kIndTheta * (nu + 2) +
+           sizeof(ProfEntry) * size_t(ntex) * kIndTheta * (nu + 1);
+}
+
+template <typename G, bool SINGLE>
 __global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
-    DevParams P, int scattering_order, const float4* __restrict__ dR,
-    const float4* __restrict__ dM, const float4* __restrict__ dMS, float4* __restrict__ dE,
-    float4* __restrict__ E) {
-    // Dynamic LDS: per wave [ntex][16][NU+2] profile slots.
+    DevParams P, const float4* __restrict__ dR, const float4* __restrict__ dM,
+    const float4* __restrict__ dMS, float4* __restrict__ dE, float4* __restrict__ E) {
     extern __shared__ float4 s_dyn[];
+    constexpr int NTEX = SINGLE ? 2 : 1;
     const int waves = blockDim.x / 64;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int texel = blockIdx.x * waves + wave;
     const int n = P.i_mus * P.i_r;
     const bool active = texel < n;
     const int ti = active ? texel % P.i_mus : 0, tj = active ? texel / P.i_mus : 0;
-    const bool single = scattering_order == 1;
-    const int NU = P.s_nu, nslots = NU + 2;
-    float4* s_prof = s_dyn + wave * (single ? 2 : 1) * kIndTheta * nslots;
+    const int NU = P.s_nu, nslots = NU + 2, nent = NU + 1;
+    const size_t per_wave = indirect_lds_bytes_per_wave(NU, SINGLE);
+    float4* s_raw = reinterpret_cast<float4*>(reinterpret_cast<char*>(s_dyn) + wave * per_wave);
+    ProfEntry* s_prof = reinterpret_cast<ProfEntry*>(s_raw + NTEX * kIndTheta * nslots);
 
     G x_mu_s = G(ti) / G(P.i_mus - 1);
     G x_r = G(tj) / G(P.i_r - 1);
     G r, mu_s;
     rmus_from_irradiance_unit<G>(P, x_mu_s, x_r, r, mu_s);
-    const G u_r = scattering_u_r<G>(P, r);
     const G u_mu_s = scattering_u_mu_s<G>(P, mu_s);
-    const Taps tz = linear_taps<G>(u_r, P.s_r);
-    const int nprof = kIndTheta * nslots * (single ? 2 : 1);
-    for (int e = lane; e < nprof; e += 64) {
+    const Taps tz = linear_taps<G>(scattering_u_r<G>(P, r), P.s_r);
+    for (int e = lane; e < NTEX * kIndTheta * nslots; e += 64) {
         int tex_sel = e / (kIndTheta * nslots);
         int rem = e % (kIndTheta * nslots);
         int jj = rem / nslots, slot = rem % nslots;
         G ct = ind_cos<G>(jj);
-        G u_mu = scattering_u_mu<G>(P, r, ct, false);
-        Taps ty = linear_taps<G>(u_mu, P.s_mu);
-        const float4* tex = single ? (tex_sel == 0 ? dR : dM) : dMS;
-        s_prof[(tex_sel * kIndTheta + jj) * nslots + slot] =
-            trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
+        Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r, ct, false), P.s_mu);
+        const float4* tex = SINGLE ? (tex_sel == 0 ? dR : dM) : dMS;
+        s_raw[e] = trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
+    }
+    __syncthreads();
+    for (int e = lane; e < NTEX * kIndTheta * nent; e += 64) {
+        int row = e / nent, k = e % nent;
+        s_prof[e] = make_entry(s_raw[row * nslots + k], s_raw[row * nslots + k + 1]);
     }
     __syncthreads();
     if (!active) return;
 
     const float osx = float(sqrt(G(1) - mu_s * mu_s)), osz = float(mu_s);
     const float g = P.g;
+    const float kR = 3.0f / (16.0f * kPi);
+    const float kM = 3.0f / (8.0f * kPi) * (1.0f - g * g) / (2.0f + g * g);
+    const float g1 = 1.0f + g * g, g2 = -2.0f * g;
+    const float hnu = 0.5f * float(NU - 1);
+    const int kmax = NU - 1;
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     for (int jj = 0; jj < kIndTheta; ++jj) {
         const float4 om = c_ind_dir[jj * kIndPhi + lane];
-        float nu = om.x * osx + om.y * 0.0f + om.z * osz;
-        float s0, s1, s2;
-        if (single) {
-            float4 vr = profile_lookup(s_prof + jj * nslots, NU, nu);
-            float4 vm = profile_lookup(s_prof + (kIndTheta + jj) * nslots, NU, nu);
-            float prr = phase_rayleigh(nu), pmm = phase_mie(g, nu);
-            s0 = vr.x * prr + vm.x * pmm;
-            s1 = vr.y * prr + vm.y * pmm;
-            s2 = vr.z * prr + vm.z * pmm;
-        } else {
-            float4 v = profile_lookup(s_prof + jj * nslots, NU, nu);
-            s0 = v.x;
-            s1 = v.y;
-            s2 = v.z;
+        const float nu = fmaf(om.x, osx, om.z * osz);
+        const float t = fmaf(nu, hnu, hnu);
+        float3 v = prof_lerp(s_prof + jj * nent, t, kmax);
+        if (SINGLE) {
+            float3 m = prof_lerp(s_prof + (kIndTheta + jj) * nent, t, kmax);
+            float q = fmaf(nu, nu, 1.0f);
+            float rs = rsqrtf(fmaf(g2, nu, g1));
+            float pR = kR * q, pM = kM * q * (rs * rs * rs);
+            v = make_float3(fmaf(m.x, pM, v.x * pR), fmaf(m.y, pM, v.y * pR),
+                            fmaf(m.z, pM, v.z * pR));
         }
-        a0 += s0 * om.z * om.w;
-        a1 += s1 * om.z * om.w;
-        a2 += s2 * om.z * om.w;
+        const float w = om.z * om.w;
+        a0 = fmaf(v.x, w, a0);
+        a1 = fmaf(v.y, w, a1);
+        a2 = fmaf(v.z, w, a2);
     }
     for (int off = 32; off > 0; off >>= 1) {
         a0 += __shfl_xor(a0, off, 64);
@@ -485,41 +605,124 @@ __global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
 }
 
 // =================================================== 6. multiple scattering
-// multiple_scattering.comp:9-93.  One thread per texel.
+// multiple_scattering.comp:9-93.
+//
+// All texels of one (r, mu) row (x = nu*MUS + mu_s varies) march the same
+// ray: d_i, r_i, mu_i, the view transmittance T(r, mu, d_i) and the
+// (u_mu, u_r) taps of every sample are shared, only mu_s_i (and the fixed
+// nu slice) differ per texel.  A block owns one row: it computes the 51
+// sample descriptors once, and for each sample stages the (mu, r)-bilinear
+// row of the density LUT in LDS (double buffered, one barrier per sample);
+// each texel then lerps 2 nu slices x 2 mu_s taps from LDS.
+constexpr int kMsTexelsPerThread = 4;
+
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
                                                              const float4* __restrict__ T,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
                                                              float4* __restrict__ S) {
-    const unsigned W = unsigned(P.s_nu * P.s_mus);
-    const size_t n = size_t(W) * P.s_mu * P.s_r;
-    size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
-    unsigned x = unsigned(idx % W), y = unsigned((idx / W) % P.s_mu),
-             z = unsigned(idx / (size_t(W) * P.s_mu));
-    TexelCoord<G> c = scattering_texel_coord<G>(P, x, y, z);
-    const int SAMPLE_COUNT = 50;
-    G dx = distance_to_nearest<G>(P, c.r, c.mu, c.ground) / G(SAMPLE_COUNT);
-    float dxf = float(dx);
-    Sum<COMP> s0, s1, s2;
-    for (int i = 0; i <= SAMPLE_COUNT; ++i) {
+    extern __shared__ float4 s_row[];           // [2][W]
+    __shared__ float4 s_tw[kMsSamples];         // T(r,mu,d_i)*dx*weight (rgb), d_i
+    __shared__ float4 s_geo[kMsSamples];        // r_i, wy, wz
+    __shared__ int4 s_off[kMsSamples];          // offsets of the 4 (z, y) rows
+    const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
+    const int W = NU * MUS;
+    const int y = blockIdx.x, z = blockIdx.y;
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    // Row geometry: r and mu depend on (y, z) only.
+    TexelCoord<G> rc = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z));
+    const G r = rc.r, mu = rc.mu;
+    const bool ground = rc.ground;
+    const G dx = distance_to_nearest<G>(P, r, mu, ground) / G(kMsSamples - 1);
+    if (tid < kMsSamples) {
+        const int i = tid;
         G d_i = G(i) * dx;
-        G r_i = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * c.r * c.mu * d_i + c.r * c.r));
-        G mu_i = clamp_cosine<G>((c.r * c.mu + d_i) / r_i);
-        G mu_s_i = clamp_cosine<G>((c.r * c.mu_s + d_i * c.nu) / r_i);
-        float4 sv = get_scattering_tex<G>(P, J, r_i, mu_i, mu_s_i, c.nu, c.ground);
-        float3 tv = get_transmittance<G>(P, T, c.r, c.mu, d_i, c.ground);
-        float w = (i == 0 || i == SAMPLE_COUNT) ? 0.5f : 1.0f;
-        s0.add(sv.x * tv.x * dxf * w);
-        s1.add(sv.y * tv.y * dxf * w);
-        s2.add(sv.z * tv.z * dxf * w);
+        G r_i = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r));
+        G mu_i = clamp_cosine<G>((r * mu + d_i) / r_i);
+        Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r_i, mu_i, ground), MU);
+        Taps tz = linear_taps<G>(scattering_u_r<G>(P, r_i), P.s_r);
+        float3 tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
+        float f = float(dx) * ((i == 0 || i == kMsSamples - 1) ? 0.5f : 1.0f);
+        s_tw[i] = make_float4(tv.x * f, tv.y * f, tv.z * f, float(d_i));
+        s_geo[i] = make_float4(float(r_i), ty.w, tz.w, 0.0f);
+        s_off[i] = make_int4((tz.i0 * MU + ty.i0) * W, (tz.i0 * MU + ty.i1) * W,
+                             (tz.i1 * MU + ty.i0) * W, (tz.i1 * MU + ty.i1) * W);
     }
-    float m0 = s0.value(), m1 = s1.value(), m2 = s2.value();
-    dMS[idx] = make_float4(m0, m1, m2, 0.0f);
-    float pr = phase_rayleigh(float(c.nu));
-    float4 prev = S[idx];
-    S[idx] = make_float4(m0 / pr + prev.x, m1 / pr + prev.y, m2 / pr + prev.z, 0.0f + prev.w);
+
+    // Per-texel constants: mu_s, nu, nu slice and its lerp weight.
+    float tmu_s[kMsTexelsPerThread], tnu[kMsTexelsPerThread], tslice[kMsTexelsPerThread],
+        tlerp[kMsTexelsPerThread];
+    Sum<COMP> acc[kMsTexelsPerThread][3];
+    const float rf = float(r);
+#pragma unroll
+    for (int q = 0; q < kMsTexelsPerThread; ++q) {
+        const int x = tid + q * nt;
+        tmu_s[q] = 0.0f;
+        tnu[q] = 0.0f;
+        tslice[q] = 0.0f;
+        tlerp[q] = 0.0f;
+        if (x < W) {
+            TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
+            tmu_s[q] = float(c.mu_s);
+            tnu[q] = float(c.nu);
+            float u_nu = (tnu[q] + 1.0f) / 2.0f;
+            float tcx = u_nu * float(NU - 1);
+            tslice[q] = floorf(tcx);
+            tlerp[q] = tcx - tslice[q];
+        }
+    }
+    __syncthreads();
+
+    auto stage = [&](int i, float4* dst) {
+        const int4 o = s_off[i];
+        const float4 gq = s_geo[i];
+        for (int x = tid; x < W; x += nt) {
+            float4 a = lerp4(ldg4(J + o.x + x), ldg4(J + o.y + x), gq.y);
+            float4 b = lerp4(ldg4(J + o.z + x), ldg4(J + o.w + x), gq.y);
+            dst[x] = lerp4(a, b, gq.z);
+        }
+    };
+    stage(0, s_row);
+    __syncthreads();
+    for (int i = 0; i < kMsSamples; ++i) {
+        const float4* cur = s_row + (i & 1) * W;
+        if (i + 1 < kMsSamples) stage(i + 1, s_row + ((i + 1) & 1) * W);
+        const float4 tw = s_tw[i];
+        const float r_i = s_geo[i].x, d_i = tw.w;
+#pragma unroll
+        for (int q = 0; q < kMsTexelsPerThread; ++q) {
+            const int x = tid + q * nt;
+            if (x < W) {
+                float mu_s_i = clamp_cosine<float>((rf * tmu_s[q] + d_i * tnu[q]) / r_i);
+                float u_mu_s = scattering_u_mu_s<float>(P, mu_s_i);
+                float x0 = (tslice[q] + u_mu_s) / float(NU);
+                float x1 = (tslice[q] + 1.0f + u_mu_s) / float(NU);
+                Taps t0 = linear_taps<float>(x0, W), t1 = linear_taps<float>(x1, W);
+                float4 v0 = lerp4(cur[t0.i0], cur[t0.i1], t0.w);
+                float4 v1 = lerp4(cur[t1.i0], cur[t1.i1], t1.w);
+                float l = tlerp[q];
+                acc[q][0].add((v0.x * (1.0f - l) + v1.x * l) * tw.x);
+                acc[q][1].add((v0.y * (1.0f - l) + v1.y * l) * tw.y);
+                acc[q][2].add((v0.z * (1.0f - l) + v1.z * l) * tw.z);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kMsTexelsPerThread; ++q) {
+        const int x = tid + q * nt;
+        if (x < W) {
+            const size_t idx = (size_t(z) * MU + y) * W + x;
+            float m0 = acc[q][0].value(), m1 = acc[q][1].value(), m2 = acc[q][2].value();
+            dMS[idx] = make_float4(m0, m1, m2, 0.0f);
+            float pr = phase_rayleigh(tnu[q]);
+            float4 prev = S[idx];
+            S[idx] = make_float4(m0 / pr + prev.x, m1 / pr + prev.y, m2 / pr + prev.z,
+                                 0.0f + prev.w);
+        }
+    }
 }
 
 // ================================================================== clear
@@ -552,11 +755,14 @@ hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4*
 
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
                                     float4* dM, float4* S, hipStream_t s) {
-    size_t n = size_t(P.s_nu) * P.s_mus * P.s_mu * P.s_r;
+    const int W = P.s_nu * P.s_mus;
+    int threads = ((W + 63) / 64) * 64;
+    threads = threads > 256 ? 256 : threads;
+    dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
-        k_single_scattering<double, true><<<cdiv(n, 256), 256, 0, s>>>(P, T, dR, dM, S);
+        k_single_scattering<double, true><<<grid, threads, 0, s>>>(P, T, dR, dM, S);
     else
-        k_single_scattering<float, false><<<cdiv(n, 256), 256, 0, s>>>(P, T, dR, dM, S);
+        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S);
     return hipGetLastError();
 }
 
@@ -564,12 +770,19 @@ hipError_t launch_scattering_density(const DevParams& P, int order, const float4
                                      const float4* dR, const float4* dM, const float4* dMS,
                                      const float4* dE, float4* J, hipStream_t s) {
     dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, 256), P.s_mus, P.s_r);
-    const int ntex = (order - 1) == 1 ? 2 : 1;
-    size_t lds = sizeof(float4) * (size_t(ntex) * kDenTheta * (P.s_nu + 2) + P.i_mus);
-    if (P.flags & 1)
-        k_scattering_density<double><<<grid, 256, lds, s>>>(P, order, T, dR, dM, dMS, dE, J);
+    // The density of order n reads the scattering of order n-1; order 1 is
+    // single Rayleigh + Mie (scattering.h:157-179).
+    const bool single = (order - 1) == 1;
+    size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
+    const bool precise = P.flags & 1;
+    if (precise && single)
+        k_scattering_density<double, true><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+    else if (precise)
+        k_scattering_density<double, false><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+    else if (single)
+        k_scattering_density<float, true><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
     else
-        k_scattering_density<float><<<grid, 256, lds, s>>>(P, order, T, dR, dM, dMS, dE, J);
+        k_scattering_density<float, false><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
     return hipGetLastError();
 }
 
@@ -577,27 +790,36 @@ hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float
                                       const float4* dM, const float4* dMS, float4* dE, float4* E,
                                       hipStream_t s) {
     size_t n = size_t(P.i_mus) * P.i_r;
-    const int ntex = order == 1 ? 2 : 1;
-    size_t per_wave = sizeof(float4) * size_t(ntex) * kIndTheta * (P.s_nu + 2);
+    const bool single = order == 1;
+    size_t per_wave = indirect_lds_bytes_per_wave(P.s_nu, single);
     int waves = int(65536 / per_wave);
     waves = waves < 1 ? 1 : (waves > kIndWaves ? kIndWaves : waves);
     size_t lds = per_wave * waves;
-    if (P.flags & 1)
-        k_indirect_irradiance<double>
-            <<<cdiv(n, waves), 64 * waves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
+    dim3 grid(cdiv(n, waves)), block(64 * waves);
+    const bool precise = P.flags & 1;
+    if (precise && single)
+        k_indirect_irradiance<double, true><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
+    else if (precise)
+        k_indirect_irradiance<double, false><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
+    else if (single)
+        k_indirect_irradiance<float, true><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
     else
-        k_indirect_irradiance<float>
-            <<<cdiv(n, waves), 64 * waves, lds, s>>>(P, order, dR, dM, dMS, dE, E);
+        k_indirect_irradiance<float, false><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
     return hipGetLastError();
 }
 
 hipError_t launch_multiple_scattering(const DevParams& P, const float4* T, const float4* J,
                                       float4* dMS, float4* S, hipStream_t s) {
-    size_t n = size_t(P.s_nu) * P.s_mus * P.s_mu * P.s_r;
+    const int W = P.s_nu * P.s_mus;
+    int threads = ((W + 63) / 64) * 64;
+    threads = threads > 256 ? 256 : threads;
+    if (W > threads * kMsTexelsPerThread) return hipErrorInvalidValue;
+    dim3 grid(P.s_mu, P.s_r);
+    size_t lds = sizeof(float4) * 2 * size_t(W);
     if (P.flags & 1)
-        k_multiple_scattering<double, true><<<cdiv(n, 256), 256, 0, s>>>(P, T, J, dMS, S);
+        k_multiple_scattering<double, true><<<grid, threads, lds, s>>>(P, T, J, dMS, S);
     else
-        k_multiple_scattering<float, false><<<cdiv(n, 256), 256, 0, s>>>(P, T, J, dMS, S);
+        k_multiple_scattering<float, false><<<grid, threads, lds, s>>>(P, T, J, dMS, S);
     return hipGetLastError();
 }
 

@@ -206,19 +206,49 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
 // scattering -> clear irradiance -> for order n in 2..=order:
 // density(n), indirect(n-1), multiple.  Stream order replaces the
 // reference's pipeline barriers.
-fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStream_t s) {
+PassTrace::~PassTrace() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+}
+
+fb_status PassTrace::mark(hipStream_t s, uint32_t pass, uint32_t order) {
+    hipEvent_t e = nullptr;
+    FB_HIP_TRY(hipEventCreate(&e));
+    events.push_back(e);
+    FB_HIP_TRY(hipEventRecord(e, s));
+    if (events.size() > 1) launches.push_back(fb_pass_timing{pass, order, 0.0f});
+    return FB_OK;
+}
+
+#define FB_MARK(pass, order)                                              \
+    do {                                                                  \
+        if (trace) {                                                      \
+            fb_status _st = trace->mark(s, (pass), uint32_t(order));      \
+            if (_st != FB_OK) return _st;                                 \
+        }                                                                 \
+    } while (0)
+
+fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStream_t s,
+                             PassTrace* trace) {
     const fbk::DevParams& P = a->dev;
     Sizes sz = sizes_of(a->params);
+    FB_MARK(0, 0);
     FB_HIP_TRY(fbk::launch_transmittance(P, a->T.p, s));
+    FB_MARK(FB_PASS_TRANSMITTANCE, 1);
     FB_HIP_TRY(fbk::launch_direct_irradiance(P, a->T.p, pa->dE.p, s));
+    FB_MARK(FB_PASS_DIRECT_IRRADIANCE, 1);
     FB_HIP_TRY(fbk::launch_single_scattering(P, a->T.p, pa->dR.p, pa->dM.p, a->S.p, s));
+    FB_MARK(FB_PASS_SINGLE_SCATTERING, 1);
     FB_HIP_TRY(fbk::launch_fill_zero(a->E.p, sz.ni, s));
+    FB_MARK(FB_PASS_CLEAR_IRRADIANCE, 1);
     for (int order = 2; order <= int(a->params.order); ++order) {
         FB_HIP_TRY(fbk::launch_scattering_density(P, order, a->T.p, pa->dR.p, pa->dM.p,
                                                   pa->dMS.p, pa->dE.p, pa->J.p, s));
+        FB_MARK(FB_PASS_SCATTERING_DENSITY, order);
         FB_HIP_TRY(fbk::launch_indirect_irradiance(P, order - 1, pa->dR.p, pa->dM.p, pa->dMS.p,
                                                    pa->dE.p, a->E.p, s));
+        FB_MARK(FB_PASS_INDIRECT_IRRADIANCE, order);
         FB_HIP_TRY(fbk::launch_multiple_scattering(P, a->T.p, pa->J.p, pa->dMS.p, a->S.p, s));
+        FB_MARK(FB_PASS_MULTIPLE_SCATTERING, order);
     }
     return FB_OK;
 }
@@ -320,6 +350,11 @@ fb_status fb_parameters_validate(const fb_parameters* p) {
     }
     if (p->scattering_nu_size > uint32_t(fbk::kMaxNu)) {
         set_error("scattering_nu_size = %u exceeds %d", p->scattering_nu_size, fbk::kMaxNu);
+        return FB_EINVAL;
+    }
+    if (p->scattering_nu_size * p->scattering_mu_s_size > uint32_t(fbk::kMaxRowTexels)) {
+        set_error("scattering_nu_size * scattering_mu_s_size = %u exceeds %d",
+                  p->scattering_nu_size * p->scattering_mu_s_size, fbk::kMaxRowTexels);
         return FB_EINVAL;
     }
     if (p->irradiance_mu_s_size > uint32_t(fbk::kMaxIrrMuS)) {
@@ -510,6 +545,30 @@ fb_status fb_pending_relaunch(fb_pending_atmosphere* pa, fb_stream stream) {
     FB_HIP_TRY(hipGraphLaunch(pa->graph, s));
     FB_HIP_TRY(hipEventRecord(pa->done, s));
     FB_HIP_TRY(hipEventRecord(pa->atm->ready, s));
+    return FB_OK;
+}
+
+fb_status fb_pending_profile(fb_pending_atmosphere* pa, fb_stream stream, fb_pass_timing* out,
+                             size_t cap, size_t* n) {
+    if (!pa || !pa->atm || !n) {
+        set_error("NULL argument or consumed pending atmosphere");
+        return FB_EINVAL;
+    }
+    FB_HIP_TRY(hipSetDevice(pa->builder->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    PassTrace trace;
+    fb_status st = enqueue_precompute(pa->atm, pa, s, &trace);
+    if (st != FB_OK) return st;
+    FB_HIP_TRY(hipEventRecord(pa->done, s));
+    FB_HIP_TRY(hipEventRecord(pa->atm->ready, s));
+    FB_HIP_TRY(hipStreamSynchronize(s));
+    *n = trace.launches.size();
+    for (size_t i = 0; i < trace.launches.size(); ++i) {
+        float ms = 0.0f;
+        FB_HIP_TRY(hipEventElapsedTime(&ms, trace.events[i], trace.events[i + 1]));
+        trace.launches[i].ms = ms;
+        if (out && i < cap) out[i] = trace.launches[i];
+    }
     return FB_OK;
 }
 

@@ -90,8 +90,16 @@ struct fb_pending_atmosphere {
 };
 
 namespace fbrt {
+// Events recorded between the launches of a schedule (fb_pending_profile).
+struct PassTrace {
+    std::vector<hipEvent_t> events;  // events[0] before the first launch
+    std::vector<fb_pass_timing> launches;
+    fb_status mark(hipStream_t s, uint32_t pass, uint32_t order);
+    ~PassTrace();
+};
 // Enqueue the whole pass schedule of one atmosphere on a stream.
-fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStream_t s);
+fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStream_t s,
+                             PassTrace* trace = nullptr);
 fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_a,
                      fb_pending_atmosphere** out_pa);
 void free_pending_scratch(fb_pending_atmosphere* pa);

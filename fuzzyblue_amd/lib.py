@@ -58,6 +58,8 @@ def load():
         "fb_atmosphere_build": (S, [V, V, P, C.POINTER(V)]),
         "fb_atmosphere_build_batch": (S, [V, V, P, C.c_size_t, V]),
         "fb_pending_relaunch": (S, [V, V]),
+        "fb_pending_profile": (
+            S, [V, V, C.POINTER(abi.fb_pass_timing), C.c_size_t, C.POINTER(C.c_size_t)]),
         "fb_pending_peek": (V, [V]),
         "fb_pending_query": (S, [V]),
         "fb_pending_wait": (S, [V]),

@@ -165,6 +165,30 @@ fb_status fb_atmosphere_build_batch(fb_builder *b, fb_stream stream,
  * benches/precompute.rs:138-148).  Uses the captured hipGraph. */
 fb_status fb_pending_relaunch(fb_pending_atmosphere *pa, fb_stream stream);
 
+/* Pass identifiers of the precompute schedule (src/precompute.rs:1726-1999). */
+typedef enum fb_pass {
+    FB_PASS_TRANSMITTANCE = 0,       /* shaders/transmittance.comp */
+    FB_PASS_DIRECT_IRRADIANCE = 1,   /* shaders/direct_irradiance.comp */
+    FB_PASS_SINGLE_SCATTERING = 2,   /* shaders/single_scattering.comp */
+    FB_PASS_CLEAR_IRRADIANCE = 3,    /* src/precompute.rs:1802-1816 */
+    FB_PASS_SCATTERING_DENSITY = 4,  /* shaders/scattering_density.comp */
+    FB_PASS_INDIRECT_IRRADIANCE = 5, /* shaders/indirect_irradiance.comp */
+    FB_PASS_MULTIPLE_SCATTERING = 6  /* shaders/multiple_scattering.comp */
+} fb_pass;
+
+typedef struct fb_pass_timing {
+    uint32_t pass;  /* fb_pass */
+    uint32_t order; /* scattering order of the launch (1 for the first passes) */
+    float ms;       /* GPU time between the events around the launch */
+} fb_pass_timing;
+
+/* Tracing: re-run the recorded schedule into the same buffers with HIP
+ * events between launches, wait, and report each launch's duration in
+ * schedule order (the reference has no timing; its harness brackets a
+ * whole submit with device_wait_idle, benches/precompute.rs:138-148). */
+fb_status fb_pending_profile(fb_pending_atmosphere *pa, fb_stream stream,
+                             fb_pass_timing *out, size_t cap, size_t *n);
+
 /* PendingAtmosphere::atmosphere (src/precompute.rs:2204): the atmosphere
  * that will be valid once the enqueued work completes. */
 const fb_atmosphere *fb_pending_peek(const fb_pending_atmosphere *pa);
