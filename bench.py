@@ -209,7 +209,19 @@ def main():
     ap.add_argument("--latency-requests", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (the known-size "
+                         "kernel that calibrates FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        src = DeviceBuffer(args.copy_calibration)
+        dst = DeviceBuffer(args.copy_calibration)
+        src.zero()
+        for _ in range(5):
+            check(load().fb_device_copy(dst.ptr, src.ptr, args.copy_calibration, None))
+        synchronize()
+        print(json.dumps({"copy_bytes": args.copy_calibration, "copies": 5}))
+        return
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: the workload is one atmosphere's LUT set on one "

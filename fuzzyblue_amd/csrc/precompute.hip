@@ -108,14 +108,20 @@ struct Sum {
 };
 
 // ============================================================ 1. transmittance
-// transmittance.comp:8-46,71-80.  One thread per texel; the three optical
-// lengths share the sample positions (the reference recomputes r_i per
-// profile with identical inputs).
-template <typename G, bool COMP>
+// transmittance.comp:8-46,71-80.  LANES lanes per texel: lane s sums the
+// trapezoid samples k = s, s+LANES, ... of all three optical lengths (the
+// reference recomputes r_i per profile with identical inputs), then the
+// partial sums are combined with lane shuffles.  The FP32 mode uses one
+// lane, i.e. the reference's sequential summation order, so its rounding
+// matches the reference's; the precision mode splits the sum over 8 lanes
+// with compensated partial sums.
+template <typename G, bool COMP, int LANES>
 __global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
-    if (i >= P.t_mu || j >= P.t_r) return;
+    const int sub = threadIdx.x % LANES;
+    const int texel = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+    const int n = P.t_mu * P.t_r;
+    const bool active = texel < n;
+    const int i = active ? texel % P.t_mu : 0, j = active ? texel / P.t_mu : 0;
     G x_mu = G(i) / G(P.t_mu - 1);
     G x_r = G(j) / G(P.t_r - 1);
     G r, mu;
@@ -124,7 +130,7 @@ __global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __re
     G dx = distance_to_top<G>(P, r, mu) / G(SAMPLE_COUNT);
     float dxf = float(dx);
     Sum<COMP> lr, lm, lo;
-    for (int k = 0; k <= SAMPLE_COUNT; ++k) {
+    for (int k = sub; k <= SAMPLE_COUNT; k += LANES) {
         G d_i = G(k) * dx;
         G r_i = sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r);
         float alt = float(r_i - G(P.bottom));
@@ -134,6 +140,12 @@ __global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __re
         lo.add(profile_density(P.prof[2], alt) * w * dxf);
     }
     float LR = lr.value(), LM = lm.value(), LO = lo.value();
+    for (int off = 1; off < LANES; off <<= 1) {
+        LR += __shfl_xor(LR, off, 64);
+        LM += __shfl_xor(LM, off, 64);
+        LO += __shfl_xor(LO, off, 64);
+    }
+    if (!active || sub != 0) return;
     float4 out;
     out.x = expf(-(P.rayleigh_scattering[0] * LR + P.mie_extinction[0] * LM +
                    P.absorption_extinction[0] * LO));
@@ -308,16 +320,25 @@ struct ProfEntry {
 
 // Linear filtering at texel coordinate t over entries e = k+1, k in
 // [-1, kmax]: entry 0 holds the clamped left edge and entry kmax+1 the
-// clamped right edge, which reproduces CLAMP_TO_EDGE.
+// clamped right edge, which reproduces CLAMP_TO_EDGE.  Callers pass a
+// finite t (their inputs are clamped unit vectors); the med3 clamp keeps
+// the entry index inside [0, kmax+1].
 __device__ __forceinline__ float3 prof_lerp(const ProfEntry* __restrict__ prof, float t,
-                                            int kmax) {
+                                            float kmax) {
     float k = floorf(t);
     float f = t - k;
-    k = fminf(fmaxf(k, -1.0f), float(kmax));
+    k = __builtin_amdgcn_fmed3f(k, -1.0f, kmax);
     const ProfEntry* e = prof + (int(k) + 1);
     float4 a = e->a;
     float2 b = e->b;
     return make_float3(fmaf(a.w, f, a.x), fmaf(b.x, f, a.y), fmaf(b.y, f, a.z));
+}
+
+// 1/x^1.5 for x >= (1-|g|)^2 > 0: the raw v_rsq_f32 (no denormal rescaling
+// is needed for this range).
+__device__ __forceinline__ float rsqrt_cubed(float x) {
+    float r = __builtin_amdgcn_rsqf(x);
+    return r * r * r;
 }
 
 __device__ __forceinline__ ProfEntry make_entry(float4 v, float4 w) {
@@ -326,6 +347,84 @@ __device__ __forceinline__ ProfEntry make_entry(float4 v, float4 w) {
     e.b = make_float2(w.y - v.y, w.z - v.z);
     e.pad = make_float2(0.0f, 0.0f);
     return e;
+}
+
+// Per-texel constants of the density quadrature.
+struct DenTexel {
+    float omx, omz;          // omega = (omx, 0, omz)
+    float osx, osy, osz;     // omega_s
+    float g1, g2;            // 1 + g^2, -2g
+    float hnu, hE;           // (NU-1)/2, (IRR_MUS-1)/2
+    int kmax, emax;          // NU-1, IRR_MUS-1
+    const ProfEntry* erow;   // delta_irradiance at r = bottom
+};
+// Per-(texel, theta) constants.
+struct DenRow {
+    float C, omzc;           // cos(theta)*omega_s.z, cos(theta)*omega.z
+    float3 rr, mm;           // kR*beta_R*rho_R*domega, kM*beta_M*rho_M*domega
+    const ProfEntry* pr;     // nu profile (order n-1 >= 2: multiple; 1: rayleigh)
+    const ProfEntry* pm;     // nu profile of single Mie (order 1)
+    const float4* dir;       // c_den_dir row: (cos phi sin t, sin phi sin t, cos t, dw)
+    float3 gv;               // T_g * albedo / pi (ground rows)
+    float Ag, Bg, Cg;        // ground normal . omega_s = cx*Ag + sy*Bg + Cg
+};
+
+// Entry address of texel coordinate t: floor clamped to [-1, kmax].
+__device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof, float t,
+                                              int kmax) {
+    int k = int(floorf(t));
+    float f = __builtin_amdgcn_fractf(t);
+    k = min(max(k, -1), kmax);
+    const ProfEntry* e = prof + (k + 1);
+    float4 a = e->a;
+    float2 b = e->b;
+    return make_float3(fmaf(a.w, f, a.x), fmaf(b.x, f, a.y), fmaf(b.y, f, a.z));
+}
+
+// The 16 direction pairs (phi_m, phi_{31-m}) of one theta row.  Both members
+// share cos(phi), hence nu2 = omega.omega_i and the phase functions; their
+// sin(phi) terms have opposite signs.
+template <bool SINGLE, bool GROUND>
+__device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& row, float3& acc) {
+#pragma unroll
+    for (int m = 0; m < kDenPhi / 2; ++m) {
+        const float4 o = row.dir[m];
+        const float cx = o.x, sy = o.y;
+        const float nu2 = fmaf(cx, dt.omx, row.omzc);
+        const float q = fmaf(nu2, nu2, 1.0f);
+        const float w = rsqrt_cubed(fmaf(dt.g2, nu2, dt.g1));
+        const float u0 = q * fmaf(row.mm.x, w, row.rr.x), u1 = q * fmaf(row.mm.y, w, row.rr.y),
+                    u2 = q * fmaf(row.mm.z, w, row.rr.z);
+        const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
+        const float nua = base + sb, nub = base - sb;
+        const float ta = fmaf(nua, dt.hnu, dt.hnu), tb = fmaf(nub, dt.hnu, dt.hnu);
+        float3 La = prof_lerp_i(row.pr, ta, dt.kmax);
+        float3 Lb = prof_lerp_i(row.pr, tb, dt.kmax);
+        if (SINGLE) {
+            // Profiles are pre-scaled by kR (rayleigh) and kM (mie).
+            const float3 Ma = prof_lerp_i(row.pm, ta, dt.kmax);
+            const float3 Mb = prof_lerp_i(row.pm, tb, dt.kmax);
+            const float qa = fmaf(nua, nua, 1.0f), qb = fmaf(nub, nub, 1.0f);
+            const float wa = rsqrt_cubed(fmaf(dt.g2, nua, dt.g1));
+            const float wb = rsqrt_cubed(fmaf(dt.g2, nub, dt.g1));
+            La = make_float3(qa * fmaf(Ma.x, wa, La.x), qa * fmaf(Ma.y, wa, La.y),
+                             qa * fmaf(Ma.z, wa, La.z));
+            Lb = make_float3(qb * fmaf(Mb.x, wb, Lb.x), qb * fmaf(Mb.y, wb, Lb.y),
+                             qb * fmaf(Mb.z, wb, Lb.z));
+        }
+        if (GROUND) {
+            const float gb = fmaf(cx, row.Ag, row.Cg), gs = sy * row.Bg;
+            const float3 Ea = prof_lerp_i(dt.erow, fmaf(gb + gs, dt.hE, dt.hE), dt.emax);
+            const float3 Eb = prof_lerp_i(dt.erow, fmaf(gb - gs, dt.hE, dt.hE), dt.emax);
+            La = make_float3(fmaf(row.gv.x, Ea.x, La.x), fmaf(row.gv.y, Ea.y, La.y),
+                             fmaf(row.gv.z, Ea.z, La.z));
+            Lb = make_float3(fmaf(row.gv.x, Eb.x, Lb.x), fmaf(row.gv.y, Eb.y, Lb.y),
+                             fmaf(row.gv.z, Eb.z, Lb.z));
+        }
+        acc.x = fmaf(La.x + Lb.x, u0, acc.x);
+        acc.y = fmaf(La.y + Lb.y, u1, acc.y);
+        acc.z = fmaf(La.z + Lb.z, u2, acc.z);
+    }
 }
 
 inline size_t density_lds_bytes(int nu, int i_mus, bool single) {
@@ -338,7 +437,7 @@ inline size_t density_lds_bytes(int nu, int i_mus, bool single) {
 // (phi_m, phi_{31-m}): both share cos(phi), hence nu2 and the two phase
 // functions; their sin(phi) terms have opposite signs.
 template <typename G, bool SINGLE>
-__global__ __launch_bounds__(256) void k_scattering_density(
+__global__ __launch_bounds__(256, 8) void k_scattering_density(
     DevParams P, const float4* __restrict__ T, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J) {
@@ -407,9 +506,18 @@ __global__ __launch_bounds__(256) void k_scattering_density(
     }
     __syncthreads();
     // 2. (value, delta) entries.
+    // Order 1 (single scattering) profiles carry the phase constants:
+    // rayleigh x 3/(16 pi), mie x 3/(8 pi)(1-g^2)/(2+g^2) (util.h:26-34).
+    const float kRs = SINGLE ? 3.0f / (16.0f * kPi) : 1.0f;
+    const float kMs =
+        SINGLE ? 3.0f / (8.0f * kPi) * (1.0f - P.g * P.g) / (2.0f + P.g * P.g) : 1.0f;
     for (int e = tid; e < NTEX * kDenTheta * nent; e += blockDim.x) {
         int row = e / nent, k = e % nent;  // entry k holds slot k-1
-        s_prof[e] = make_entry(s_raw[row * nslots + k], s_raw[row * nslots + k + 1]);
+        float sc = (SINGLE && row >= kDenTheta) ? kMs : kRs;
+        float4 v = s_raw[row * nslots + k], w = s_raw[row * nslots + k + 1];
+        v = make_float4(v.x * sc, v.y * sc, v.z * sc, 0.0f);
+        w = make_float4(w.x * sc, w.y * sc, w.z * sc, 0.0f);
+        s_prof[e] = make_entry(v, w);
     }
     for (int e = tid; e <= NE; e += blockDim.x) {
         int i0 = min(max(e - 1, 0), NE - 1), i1 = min(e, NE - 1);
@@ -427,87 +535,60 @@ __global__ __launch_bounds__(256) void k_scattering_density(
     G om_x = sqrt(G(1) - c.mu * c.mu);
     G sdx = om_x == G(0) ? G(0) : (c.nu - c.mu * c.mu_s) / om_x;
     G sdy = sqrt(fmax(G(1) - sdx * sdx - c.mu_s * c.mu_s, G(0)));
-    const float omx = float(om_x), omz = float(c.mu);
-    const float osx = float(sdx), osy = float(sdy), osz = float(c.mu_s);
+    DenTexel dt;
+    dt.omx = float(om_x);
+    dt.omz = float(c.mu);
+    // Unit-vector components, clamped so every lookup coordinate is finite.
+    dt.osx = fminf(fmaxf(float(sdx), -1.0f), 1.0f);
+    dt.osy = fminf(fmaxf(float(sdy), -1.0f), 1.0f);
+    dt.osz = float(c.mu_s);
     const float rf = float(r);
     const float alt = float(c.r - G(P.bottom));
     const float rho_r = profile_density(P.prof[0], alt);
     const float rho_m = profile_density(P.prof[1], alt);
-    const float cr0 = P.rayleigh_scattering[0] * rho_r, cr1 = P.rayleigh_scattering[1] * rho_r,
-                cr2 = P.rayleigh_scattering[2] * rho_r;
-    const float cm0 = P.mie_scattering[0] * rho_m, cm1 = P.mie_scattering[1] * rho_m,
-                cm2 = P.mie_scattering[2] * rho_m;
-    const int gmask = s_ground_mask;
     const float g = P.g;
     const float kR = 3.0f / (16.0f * kPi);
     const float kM = 3.0f / (8.0f * kPi) * (1.0f - g * g) / (2.0f + g * g);
-    const float g1 = 1.0f + g * g, g2 = -2.0f * g;
-    const float hnu = 0.5f * float(NU - 1), hE = 0.5f * float(NE - 1);
-    const int kmax = NU - 1, emax = NE - 1;
+    const float cr0 = kR * P.rayleigh_scattering[0] * rho_r,
+                cr1 = kR * P.rayleigh_scattering[1] * rho_r,
+                cr2 = kR * P.rayleigh_scattering[2] * rho_r;
+    const float cm0 = kM * P.mie_scattering[0] * rho_m, cm1 = kM * P.mie_scattering[1] * rho_m,
+                cm2 = kM * P.mie_scattering[2] * rho_m;
+    const int gmask = s_ground_mask;
+    dt.g1 = 1.0f + g * g;
+    dt.g2 = -2.0f * g;
+    dt.hnu = 0.5f * float(NU - 1);
+    dt.hE = 0.5f * float(NE - 1);
+    dt.kmax = NU - 1;
+    dt.emax = NE - 1;
+    dt.erow = s_erow;
 
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    float3 acc = make_float3(0.0f, 0.0f, 0.0f);
     for (int l = 0; l < kDenTheta; ++l) {
         const float4 d0 = c_den_dir[l * kDenPhi];
         const float ct = d0.z, dom = d0.w;
-        const float C = ct * osz, omzc = ct * omz;
-        const float r0 = cr0 * dom, r1 = cr1 * dom, r2 = cr2 * dom;
-        const float m0 = cm0 * dom, m1 = cm1 * dom, m2 = cm2 * dom;
-        const ProfEntry* pr = s_prof + l * nent;
-        const ProfEntry* pm = s_prof + (kDenTheta + l) * nent;
-        const bool ground_l = (gmask >> l) & 1;
-        float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float Ag = 0.0f, Bg = 0.0f, Cg = 0.0f;
-        if (ground_l) {
-            gv = s_gv[l];
-            float inv = s_ginv[l];
-            float dgi = gv.w * inv;
-            Ag = dgi * osx;
-            Bg = dgi * osy;
-            Cg = (rf + ct * gv.w) * inv * osz;
-        }
-#pragma unroll 2
-        for (int m = 0; m < kDenPhi / 2; ++m) {
-            const float4 o = c_den_dir[l * kDenPhi + m];
-            const float cx = o.x, sy = o.y;
-            // Shared: phase functions at nu2 = omega . omega_i.
-            const float nu2 = fmaf(cx, omx, omzc);
-            const float q = fmaf(nu2, nu2, 1.0f);
-            const float rs = rsqrtf(fmaf(g2, nu2, g1));
-            const float pR = kR * q, pM = kM * q * (rs * rs * rs);
-            const float s0 = fmaf(m0, pM, r0 * pR), s1 = fmaf(m1, pM, r1 * pR),
-                        s2 = fmaf(m2, pM, r2 * pR);
-            // The pair's incident radiance.
-            const float base = fmaf(cx, osx, C), sb = sy * osy;
-            const float nua = base + sb, nub = base - sb;
-            float3 La = prof_lerp(pr, fmaf(nua, hnu, hnu), kmax);
-            float3 Lb = prof_lerp(pr, fmaf(nub, hnu, hnu), kmax);
-            if (SINGLE) {
-                float3 Ma = prof_lerp(pm, fmaf(nua, hnu, hnu), kmax);
-                float3 Mb = prof_lerp(pm, fmaf(nub, hnu, hnu), kmax);
-                float qa = fmaf(nua, nua, 1.0f), qb = fmaf(nub, nub, 1.0f);
-                float ra = rsqrtf(fmaf(g2, nua, g1)), rb = rsqrtf(fmaf(g2, nub, g1));
-                float pRa = kR * qa, pMa = kM * qa * (ra * ra * ra);
-                float pRb = kR * qb, pMb = kM * qb * (rb * rb * rb);
-                La = make_float3(fmaf(Ma.x, pMa, La.x * pRa), fmaf(Ma.y, pMa, La.y * pRa),
-                                 fmaf(Ma.z, pMa, La.z * pRa));
-                Lb = make_float3(fmaf(Mb.x, pMb, Lb.x * pRb), fmaf(Mb.y, pMb, Lb.y * pRb),
-                                 fmaf(Mb.z, pMb, Lb.z * pRb));
-            }
-            if (ground_l) {
-                const float gb = fmaf(cx, Ag, Cg), gs = sy * Bg;
-                float3 Ea = prof_lerp(s_erow, fmaf(gb + gs, hE, hE), emax);
-                float3 Eb = prof_lerp(s_erow, fmaf(gb - gs, hE, hE), emax);
-                La = make_float3(fmaf(gv.x, Ea.x, La.x), fmaf(gv.y, Ea.y, La.y),
-                                 fmaf(gv.z, Ea.z, La.z));
-                Lb = make_float3(fmaf(gv.x, Eb.x, Lb.x), fmaf(gv.y, Eb.y, Lb.y),
-                                 fmaf(gv.z, Eb.z, Lb.z));
-            }
-            a0 = fmaf(La.x + Lb.x, s0, a0);
-            a1 = fmaf(La.y + Lb.y, s1, a1);
-            a2 = fmaf(La.z + Lb.z, s2, a2);
+        DenRow row;
+        row.C = ct * dt.osz;
+        row.omzc = ct * dt.omz;
+        row.rr = make_float3(cr0 * dom, cr1 * dom, cr2 * dom);
+        row.mm = make_float3(cm0 * dom, cm1 * dom, cm2 * dom);
+        row.pr = s_prof + l * nent;
+        row.pm = s_prof + (kDenTheta + l) * nent;
+        row.dir = c_den_dir + l * kDenPhi;
+        if ((gmask >> l) & 1) {
+            const float4 gv = s_gv[l];
+            const float inv = s_ginv[l];
+            const float dgi = gv.w * inv;
+            row.gv = make_float3(gv.x, gv.y, gv.z);
+            row.Ag = dgi * dt.osx;
+            row.Bg = dgi * dt.osy;
+            row.Cg = (rf + ct * gv.w) * inv * dt.osz;
+            density_row<SINGLE, true>(dt, row, acc);
+        } else {
+            density_row<SINGLE, false>(dt, row, acc);
         }
     }
-    J[(size_t(z) * MU + y) * W + x] = make_float4(a0, a1, a2, 0.0f);
+    J[(size_t(z) * MU + y) * W + x] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }
 
 
@@ -624,7 +705,7 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
                                                              float4* __restrict__ S) {
     extern __shared__ float4 s_row[];           // [2][W]
     __shared__ float4 s_tw[kMsSamples];         // T(r,mu,d_i)*dx*weight (rgb), d_i
-    __shared__ float4 s_geo[kMsSamples];        // r_i, wy, wz
+    __shared__ float4 s_geo[kMsSamples];        // 1/r_i, wy, wz
     __shared__ int4 s_off[kMsSamples];          // offsets of the 4 (z, y) rows
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
@@ -646,7 +727,7 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
         float3 tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
         float f = float(dx) * ((i == 0 || i == kMsSamples - 1) ? 0.5f : 1.0f);
         s_tw[i] = make_float4(tv.x * f, tv.y * f, tv.z * f, float(d_i));
-        s_geo[i] = make_float4(float(r_i), ty.w, tz.w, 0.0f);
+        s_geo[i] = make_float4(float(G(1) / r_i), ty.w, tz.w, 0.0f);
         s_off[i] = make_int4((tz.i0 * MU + ty.i0) * W, (tz.i0 * MU + ty.i1) * W,
                              (tz.i1 * MU + ty.i0) * W, (tz.i1 * MU + ty.i1) * W);
     }
@@ -675,13 +756,28 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
     }
     __syncthreads();
 
+    // GetScatteringTextureUvwzFromRMuMuSNu's mu_s coordinate (scattering.h:
+    // 48-56) folded into the texel coordinate of slice `tslice`:
+    // x_tex = x_mu_s*(MUS-1) + tslice*MUS, with reciprocals hoisted.
+    const float b = P.bottom, b2 = P.bottom * P.bottom;
+    const float H2 = P.top * P.top - P.bottom * P.bottom;
+    const float d_min = P.top - P.bottom;
+    const float inv_range = 1.0f / (sqrtf(H2) - d_min);
+    const float invA = 1.0f / (-2.0f * P.mu_s_min * P.bottom * inv_range);
+    const float musm1 = float(MUS - 1), fMUS = float(MUS);
+
     auto stage = [&](int i, float4* dst) {
         const int4 o = s_off[i];
         const float4 gq = s_geo[i];
         for (int x = tid; x < W; x += nt) {
-            float4 a = lerp4(ldg4(J + o.x + x), ldg4(J + o.y + x), gq.y);
-            float4 b = lerp4(ldg4(J + o.z + x), ldg4(J + o.w + x), gq.y);
-            dst[x] = lerp4(a, b, gq.z);
+            float4 r00 = ldg4(J + o.x + x), r01 = ldg4(J + o.y + x);
+            float4 r10 = ldg4(J + o.z + x), r11 = ldg4(J + o.w + x);
+            float ax = fmaf(gq.y, r01.x - r00.x, r00.x), ay = fmaf(gq.y, r01.y - r00.y, r00.y),
+                  az = fmaf(gq.y, r01.z - r00.z, r00.z);
+            float bx = fmaf(gq.y, r11.x - r10.x, r10.x), by = fmaf(gq.y, r11.y - r10.y, r10.y),
+                  bz = fmaf(gq.y, r11.z - r10.z, r10.z);
+            dst[x] = make_float4(fmaf(gq.z, bx - ax, ax), fmaf(gq.z, by - ay, ay),
+                                 fmaf(gq.z, bz - az, az), 0.0f);
         }
     };
     stage(0, s_row);
@@ -690,22 +786,33 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
         const float4* cur = s_row + (i & 1) * W;
         if (i + 1 < kMsSamples) stage(i + 1, s_row + ((i + 1) & 1) * W);
         const float4 tw = s_tw[i];
-        const float r_i = s_geo[i].x, d_i = tw.w;
+        const float inv_r_i = s_geo[i].x, d_i = tw.w;
 #pragma unroll
         for (int q = 0; q < kMsTexelsPerThread; ++q) {
             const int x = tid + q * nt;
             if (x < W) {
-                float mu_s_i = clamp_cosine<float>((rf * tmu_s[q] + d_i * tnu[q]) / r_i);
-                float u_mu_s = scattering_u_mu_s<float>(P, mu_s_i);
-                float x0 = (tslice[q] + u_mu_s) / float(NU);
-                float x1 = (tslice[q] + 1.0f + u_mu_s) / float(NU);
-                Taps t0 = linear_taps<float>(x0, W), t1 = linear_taps<float>(x1, W);
-                float4 v0 = lerp4(cur[t0.i0], cur[t0.i1], t0.w);
-                float4 v1 = lerp4(cur[t1.i0], cur[t1.i1], t1.w);
+                // mu_s at the sample (multiple_scattering.comp:38), in [-1, 1].
+                float mu_s_i = fmaf(rf, tmu_s[q], d_i * tnu[q]) * inv_r_i;
+                mu_s_i = __builtin_amdgcn_fmed3f(mu_s_i, -1.0f, 1.0f);
+                float disc = fmaf(b2, mu_s_i * mu_s_i, H2);
+                float d = fmaxf(fmaf(-b, mu_s_i, __builtin_amdgcn_sqrtf(disc)), 0.0f);
+                float a = (d - d_min) * inv_range;
+                float xm = fmaxf(fmaf(-a, invA, 1.0f), 0.0f) * __builtin_amdgcn_rcpf(1.0f + a);
+                float xt = fmaf(xm, musm1, tslice[q] * fMUS);
+                float fl = floorf(xt);
+                float w = xt - fl;
+                int i0 = int(fl);
+                int c0 = min(max(i0, 0), W - 1), c1 = min(max(i0 + 1, 0), W - 1);
+                int c2 = min(max(i0 + MUS, 0), W - 1), c3 = min(max(i0 + MUS + 1, 0), W - 1);
+                float4 p0 = cur[c0], p1 = cur[c1], p2 = cur[c2], p3 = cur[c3];
                 float l = tlerp[q];
-                acc[q][0].add((v0.x * (1.0f - l) + v1.x * l) * tw.x);
-                acc[q][1].add((v0.y * (1.0f - l) + v1.y * l) * tw.y);
-                acc[q][2].add((v0.z * (1.0f - l) + v1.z * l) * tw.z);
+                float v0 = fmaf(w, p1.x - p0.x, p0.x), v1 = fmaf(w, p1.y - p0.y, p0.y),
+                      v2 = fmaf(w, p1.z - p0.z, p0.z);
+                float u0 = fmaf(w, p3.x - p2.x, p2.x), u1 = fmaf(w, p3.y - p2.y, p2.y),
+                      u2 = fmaf(w, p3.z - p2.z, p2.z);
+                acc[q][0].add(fmaf(l, u0 - v0, v0) * tw.x);
+                acc[q][1].add(fmaf(l, u1 - v1, v1) * tw.y);
+                acc[q][2].add(fmaf(l, u2 - v2, v2) * tw.z);
             }
         }
         __syncthreads();
@@ -735,11 +842,11 @@ __global__ void k_fill_zero(float4* __restrict__ p, size_t n) {
 static inline unsigned cdiv(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
 
 hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s) {
-    dim3 grid(cdiv(P.t_mu, 64), P.t_r);
+    size_t texels = size_t(P.t_mu) * P.t_r;
     if (P.flags & 1)
-        k_transmittance<double, true><<<grid, 64, 0, s>>>(P, T);
+        k_transmittance<double, true, 8><<<cdiv(texels * 8, 256), 256, 0, s>>>(P, T);
     else
-        k_transmittance<float, false><<<grid, 64, 0, s>>>(P, T);
+        k_transmittance<float, false, 1><<<cdiv(texels, 64), 64, 0, s>>>(P, T);
     return hipGetLastError();
 }
 

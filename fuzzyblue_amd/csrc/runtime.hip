@@ -83,6 +83,17 @@ void Pool::trim() {
     cached_ = 0;
 }
 
+void builder_retain(fb_builder* b) { b->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void builder_release(fb_builder* b) {
+    if (b->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    b->pool.reset();
+    delete b;
+}
+
 // ------------------------------------------------------------ parameters
 Sizes sizes_of(const fb_parameters& p) {
     Sizes s;
@@ -167,9 +178,11 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
     auto* a = new fb_atmosphere();
     auto* pa = new fb_pending_atmosphere();
     a->builder = b;
+    builder_retain(b);
     a->params = p;
     a->dev = make_dev_params(p);
     pa->builder = b;
+    builder_retain(b);
     pa->atm = a;
     fb_status st = FB_OK;
     if (st == FB_OK) st = alloc_buf(b, s.nt, &a->T);
@@ -194,6 +207,8 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
         if (a->last_use) (void)hipEventDestroy(a->last_use);
         delete pa;
         delete a;
+        builder_release(b);
+        builder_release(b);
         return st;
     }
     *out_a = a;
@@ -471,11 +486,7 @@ fb_status fb_builder_create(int hip_device, fb_builder** out) {
 
 void fb_builder_destroy(fb_builder* b) {
     if (!b) return;
-    (void)hipSetDevice(b->device);
-    (void)hipDeviceSynchronize();
-    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
-    b->pool.reset();
-    delete b;
+    builder_release(b);
 }
 
 // ------------------------------------------------------------- precompute
@@ -509,6 +520,8 @@ fb_status fb_atmosphere_build(fb_builder* b, fb_stream stream, const fb_paramete
         (void)hipEventDestroy(a->last_use);
         delete pa;
         delete a;
+        builder_release(b);
+        builder_release(b);
         return st;
     }
     *out = pa;
@@ -633,7 +646,9 @@ void fb_pending_destroy(fb_pending_atmosphere* pa) {
     free_pending_scratch(pa);
     (void)hipEventDestroy(pa->done);
     if (pa->atm) fb_atmosphere_destroy(pa->atm);
+    fb_builder* b = pa->builder;
     delete pa;
+    builder_release(b);
 }
 
 // --------------------------------------------------------------- accessors
@@ -752,7 +767,9 @@ void fb_atmosphere_destroy(fb_atmosphere* a) {
     free_atmosphere_buffers(a);
     (void)hipEventDestroy(a->ready);
     (void)hipEventDestroy(a->last_use);
+    fb_builder* b = a->builder;
     delete a;
+    builder_release(b);
 }
 
 // ----------------------------------------------------------------- plumbing
@@ -787,6 +804,11 @@ fb_status fb_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 }
 fb_status fb_memset_d(void* dst, int value, size_t bytes) {
     FB_HIP_TRY(hipMemset(dst, value, bytes));
+    return FB_OK;
+}
+fb_status fb_device_copy(void* dst, const void* src, size_t bytes, fb_stream stream) {
+    FB_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                              static_cast<hipStream_t>(stream)));
     return FB_OK;
 }
 fb_status fb_stream_create(fb_stream* out) {

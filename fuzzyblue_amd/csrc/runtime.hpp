@@ -62,13 +62,24 @@ Sizes sizes_of(const fb_parameters& p);
 }  // namespace fbrt
 
 // Opaque C-ABI objects.
+//
+// The builder is reference counted like the reference's Arc<Builder>
+// (src/precompute.rs:1045-1047, 1077): every atmosphere and pending
+// atmosphere holds a reference, fb_builder_destroy drops the caller's, and
+// the device state is released with the last one.
 struct fb_builder {
     int device = 0;
     hipDeviceProp_t prop{};
     std::unique_ptr<fbrt::Pool> pool;
     hipStream_t capture_stream = nullptr;  // private stream for graph capture
     std::mutex mu;
+    std::atomic<int> refs{1};
 };
+
+namespace fbrt {
+void builder_retain(fb_builder* b);
+void builder_release(fb_builder* b);
+}  // namespace fbrt
 
 struct fb_atmosphere {
     fb_builder* builder = nullptr;

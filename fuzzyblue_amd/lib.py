@@ -109,6 +109,7 @@ def load():
         "fb_memcpy_h2d": (S, [V, V, C.c_size_t]),
         "fb_memcpy_d2h": (S, [V, V, C.c_size_t]),
         "fb_memset_d": (S, [V, S, C.c_size_t]),
+        "fb_device_copy": (S, [V, V, C.c_size_t, V]),
         "fb_stream_create": (S, [C.POINTER(V)]),
         "fb_stream_destroy": (S, [V]),
         "fb_stream_synchronize": (S, [V]),

@@ -394,6 +394,10 @@ fb_status fb_device_free(void *ptr);
 fb_status fb_memcpy_h2d(void *dst, const void *src, size_t bytes);
 fb_status fb_memcpy_d2h(void *dst, const void *src, size_t bytes);
 fb_status fb_memset_d(void *dst, int value, size_t bytes);
+/* Device-to-device copy enqueued on a stream (also the known-size kernel
+ * that calibrates HBM byte counters). */
+fb_status fb_device_copy(void *dst, const void *src, size_t bytes,
+                         fb_stream stream);
 fb_status fb_stream_create(fb_stream *out);
 fb_status fb_stream_destroy(fb_stream s);
 fb_status fb_stream_synchronize(fb_stream s);
