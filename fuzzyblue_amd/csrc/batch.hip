@@ -1,7 +1,222 @@
-// batch.hip — many atmospheres in one build call.
+// batch.hip — many atmospheres of uneven LUT resolution and scattering
+// order in one launch per pass.
+//
+// The reference builds one atmosphere per recorded command buffer
+// (src/precompute.rs:1077-2073); here a batch plan enqueues, for every pass
+// of the schedule (src/precompute.rs:1726-1999), ONE launch whose blocks
+// cover every participating atmosphere: block b reads tile word b
+// (atmosphere slot << 24 | block index inside that atmosphere's grid) and
+// finds the atmosphere's parameters and LUT pointers in HBM arrays.  The
+// order loop runs to the largest order in the batch; an atmosphere of order
+// k contributes no tiles to the launches of orders above k.  The whole
+// schedule is captured once as a hipGraph and replayed.
+#include <algorithm>
+#include <memory>
+#include <vector>
+
 #include "runtime.hpp"
 
 using namespace fbrt;
+
+namespace {
+
+struct BatchLaunch {
+    int pass;
+    int order;
+    uint32_t* d_tiles = nullptr;
+    unsigned ntiles = 0;
+    unsigned threads = 0;
+    size_t lds = 0;
+};
+
+// Shared state of one batch: device arrays, the launch list and the graph.
+// Members (pending atmospheres) hold it through shared_ptr; it is destroyed
+// with the last member, after that member waited for the batch's work.
+struct BatchPlan {
+    fb_builder* builder = nullptr;
+    bool precise = false;
+    std::mutex mu;
+    size_t n_total = 0;
+    std::vector<fb_pending_atmosphere*> members;
+    fbk::DevParams* d_params = nullptr;
+    fbk::LutPtrs* d_luts = nullptr;
+    std::vector<BatchLaunch> launches;
+    hipGraphExec_t graph = nullptr;
+
+    ~BatchPlan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& l : launches)
+            if (l.d_tiles) (void)hipFree(l.d_tiles);
+        if (d_params) (void)hipFree(d_params);
+        if (d_luts) (void)hipFree(d_luts);
+    }
+
+    fb_status enqueue(hipStream_t s) const {
+        for (const auto& l : launches)
+            FB_HIP_TRY(fbk::launch_batch(l.pass, l.order, precise, d_params, d_luts, l.d_tiles,
+                                         l.ntiles, l.threads, l.lds, s));
+        return FB_OK;
+    }
+};
+
+// Order the batch's tiles so that each atmosphere's blocks are contiguous
+// (its LUTs stay hot in L2 while its blocks run) and large atmospheres come
+// first (the small ones fill the tail of the launch).
+std::vector<int> tile_order(const std::vector<fb_parameters>& ps) {
+    std::vector<int> idx(ps.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = int(i);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+        return sizes_of(ps[a]).ns > sizes_of(ps[b]).ns;
+    });
+    return idx;
+}
+
+fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
+                     const std::vector<fbk::DevParams>& dps, const std::vector<int>& order_idx,
+                     int pass, int order) {
+    BatchLaunch l;
+    l.pass = pass;
+    l.order = order;
+    std::vector<uint32_t> tiles;
+    for (int i : order_idx) {
+        if (pass >= fbk::kPassDensity && int(ps[i].order) < order) continue;
+        fbk::PassShape sh = fbk::batch_pass_shape(dps[i], pass, order);
+        if (sh.blocks >= (1u << 24)) {
+            set_error("atmosphere %d needs %u blocks in pass %d (limit 2^24)", i, sh.blocks, pass);
+            return FB_EINVAL;
+        }
+        for (unsigned b = 0; b < sh.blocks; ++b) tiles.push_back((uint32_t(i) << 24) | b);
+        l.threads = std::max(l.threads, sh.threads);
+        l.lds = std::max(l.lds, sh.lds);
+    }
+    if (tiles.empty()) return FB_OK;
+    if (l.lds > 65536) {
+        set_error("pass %d needs %zu bytes of LDS per block", pass, l.lds);
+        return FB_EINVAL;
+    }
+    l.ntiles = unsigned(tiles.size());
+    FB_HIP_TRY(hipMalloc(&l.d_tiles, tiles.size() * sizeof(uint32_t)));
+    plan.launches.push_back(l);  // owned by the plan from here on
+    FB_HIP_TRY(hipMemcpy(plan.launches.back().d_tiles, tiles.data(),
+                         tiles.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return FB_OK;
+}
+
+// Build (and launch) one group: same precision flags, at most 256 members.
+fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_parameters>& ps,
+                      fb_pending_atmosphere** out) {
+    auto plan = std::make_shared<BatchPlan>();
+    plan->builder = b;
+    plan->precise = (ps[0].flags & FB_FLAG_PRECISE) != 0;
+    const size_t n = ps.size();
+    std::vector<fbk::DevParams> dps(n);
+    std::vector<fbk::LutPtrs> luts(n);
+    fb_status st = FB_OK;
+    for (size_t i = 0; i < n && st == FB_OK; ++i) {
+        fb_atmosphere* a = nullptr;
+        fb_pending_atmosphere* pa = nullptr;
+        st = alloc_luts(b, ps[i], &a, &pa);
+        if (st != FB_OK) break;
+        plan->members.push_back(pa);
+        dps[i] = a->dev;
+        luts[i] = fbk::LutPtrs{a->T.p, a->E.p, a->S.p, pa->dE.p, pa->dR.p, pa->dM.p, pa->dMS.p,
+                               pa->J.p};
+    }
+    auto fail = [&](fb_status code) {
+        for (auto* pa : plan->members) fb_pending_destroy(pa);
+        plan->members.clear();
+        return code;
+    };
+    if (st != FB_OK) return fail(st);
+    hipError_t e = hipMalloc(&plan->d_params, n * sizeof(fbk::DevParams));
+    if (e == hipSuccess) e = hipMalloc(&plan->d_luts, n * sizeof(fbk::LutPtrs));
+    if (e == hipSuccess)
+        e = hipMemcpy(plan->d_params, dps.data(), n * sizeof(fbk::DevParams),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(plan->d_luts, luts.data(), n * sizeof(fbk::LutPtrs), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(hip_fail(e, "batch parameter upload"));
+
+    const std::vector<int> order_idx = tile_order(ps);
+    int max_order = 1;
+    for (const auto& p : ps) max_order = std::max(max_order, int(p.order));
+    for (int pass : {fbk::kPassTransmittance, fbk::kPassDirect, fbk::kPassSingle,
+                     fbk::kPassClear})
+        if ((st = add_launch(*plan, ps, dps, order_idx, pass, 1)) != FB_OK) return fail(st);
+    for (int order = 2; order <= max_order; ++order)
+        for (int pass : {fbk::kPassDensity, fbk::kPassIndirect, fbk::kPassMultiple})
+            if ((st = add_launch(*plan, ps, dps, order_idx, pass, order)) != FB_OK)
+                return fail(st);
+
+    // Capture the schedule once; replays reuse it (fb_pending_relaunch).
+    {
+        std::lock_guard<std::mutex> g(b->mu);
+        hipGraph_t graph = nullptr;
+        e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return fail(hip_fail(e, "hipStreamBeginCapture"));
+        st = plan->enqueue(b->capture_stream);
+        e = hipStreamEndCapture(b->capture_stream, &graph);
+        if (st != FB_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(st);
+        }
+        if (e != hipSuccess) return fail(hip_fail(e, "hipStreamEndCapture"));
+        e = hipGraphInstantiate(&plan->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            plan->graph = nullptr;
+            return fail(hip_fail(e, "hipGraphInstantiate"));
+        }
+    }
+    e = hipGraphLaunch(plan->graph, s);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipGraphLaunch"));
+    plan->n_total = plan->members.size();
+    for (auto* pa : plan->members) {
+        e = hipEventRecord(pa->done, s);
+        if (e == hipSuccess) e = hipEventRecord(pa->atm->ready, s);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(s);
+            return fail(hip_fail(e, "hipEventRecord"));
+        }
+        pa->batch = plan;
+    }
+    for (size_t i = 0; i < n; ++i) out[i] = plan->members[i];
+    return FB_OK;
+}
+
+}  // namespace
+
+namespace fbrt {
+// Replay a batch member's captured schedule (the whole batch).  Only while
+// every member is alive and unconsumed: the graph writes every member's
+// scratch and output LUTs.
+fb_status batch_relaunch(fb_pending_atmosphere* pa, hipStream_t s) {
+    auto plan = std::static_pointer_cast<BatchPlan>(pa->batch);
+    std::lock_guard<std::mutex> g(plan->mu);
+    if (plan->members.size() != plan->n_total) {
+        set_error("batch has released members; it can no longer be relaunched");
+        return FB_EBUSY;
+    }
+    for (auto* m : plan->members)
+        if (!m->atm) {
+            set_error("a batch member was consumed by assert_ready");
+            return FB_EBUSY;
+        }
+    FB_HIP_TRY(hipGraphLaunch(plan->graph, s));
+    for (auto* m : plan->members) {
+        FB_HIP_TRY(hipEventRecord(m->done, s));
+        if (m->atm) FB_HIP_TRY(hipEventRecord(m->atm->ready, s));
+    }
+    return FB_OK;
+}
+// Called by fb_pending_destroy: the member leaves the batch.
+void batch_detach(fb_pending_atmosphere* pa) {
+    auto plan = std::static_pointer_cast<BatchPlan>(pa->batch);
+    std::lock_guard<std::mutex> g(plan->mu);
+    plan->members.erase(std::remove(plan->members.begin(), plan->members.end(), pa),
+                        plan->members.end());
+}
+}  // namespace fbrt
 
 extern "C" fb_status fb_atmosphere_build_batch(fb_builder* b, fb_stream stream,
                                                const fb_parameters* params, size_t n,
@@ -13,16 +228,34 @@ extern "C" fb_status fb_atmosphere_build_batch(fb_builder* b, fb_stream stream,
     for (size_t i = 0; i < n; ++i) {
         fb_status st = fb_parameters_validate(&params[i]);
         if (st != FB_OK) return st;
+        out[i] = nullptr;
     }
-    for (size_t i = 0; i < n; ++i) out[i] = nullptr;
-    for (size_t i = 0; i < n; ++i) {
-        fb_status st = fb_atmosphere_build(b, stream, &params[i], &out[i]);
-        if (st != FB_OK) {
-            for (size_t k = 0; k < i; ++k) {
-                fb_pending_destroy(out[k]);
-                out[k] = nullptr;
+    FB_HIP_TRY(hipSetDevice(b->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // Groups: one per precision mode, at most 256 atmospheres each (the tile
+    // word's 8-bit atmosphere slot).
+    std::vector<size_t> built;
+    for (uint32_t flags : {0u, FB_FLAG_PRECISE}) {
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < n; ++i)
+            if ((params[i].flags & FB_FLAG_PRECISE) == flags) idx.push_back(i);
+        for (size_t g0 = 0; g0 < idx.size(); g0 += 256) {
+            size_t g1 = std::min(idx.size(), g0 + 256);
+            std::vector<fb_parameters> ps;
+            for (size_t k = g0; k < g1; ++k) ps.push_back(params[idx[k]]);
+            std::vector<fb_pending_atmosphere*> outs(ps.size(), nullptr);
+            fb_status st = build_group(b, s, ps, outs.data());
+            if (st != FB_OK) {
+                for (size_t i : built) {
+                    fb_pending_destroy(out[i]);
+                    out[i] = nullptr;
+                }
+                return st;
             }
-            return st;
+            for (size_t k = g0; k < g1; ++k) {
+                out[idx[k]] = outs[k - g0];
+                built.push_back(idx[k]);
+            }
         }
     }
     return FB_OK;

@@ -23,6 +23,40 @@ constexpr int kMaxRowTexels = 1024;  // NU * MUS: one scattering row per block
 
 hipError_t upload_tables();
 
+// Device addresses of one atmosphere's LUTs (batched launches read an HBM
+// array of these, indexed by the tile's atmosphere slot).
+struct LutPtrs {
+    float4 *T, *E, *S, *dE, *dR, *dM, *dMS, *J;
+};
+
+// Passes of a batched schedule (values match fb_pass).
+enum BatchPass {
+    kPassTransmittance = 0,
+    kPassDirect = 1,
+    kPassSingle = 2,
+    kPassClear = 3,
+    kPassDensity = 4,
+    kPassIndirect = 5,
+    kPassMultiple = 6
+};
+
+// Launch shape of one atmosphere's share of a batched pass: the number of
+// blocks (= tiles), threads per block and dynamic LDS bytes.
+struct PassShape {
+    unsigned blocks;
+    unsigned threads;
+    size_t lds;
+};
+PassShape batch_pass_shape(const DevParams& P, int pass, int order);
+
+// One launch of `pass` over a tile list (atmosphere slot << 24 | block).
+// `order` is the scattering order of the launch (density n, indirect n-1's
+// caller order n, multiple n); threads / lds are the maxima over the tiles'
+// atmospheres.
+hipError_t launch_batch(int pass, int order, bool precise, const DevParams* d_params,
+                        const LutPtrs* d_luts, const uint32_t* d_tiles, unsigned ntiles,
+                        unsigned threads, size_t lds, hipStream_t s);
+
 hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s);
 hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,
                                     hipStream_t s);

@@ -116,9 +116,9 @@ struct Sum {
 // matches the reference's; the precision mode splits the sum over 8 lanes
 // with compensated partial sums.
 template <typename G, bool COMP, int LANES>
-__global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T) {
+__device__ __forceinline__ void transmittance_block(const DevParams& P, const int bx, const int by, const int bz, float4* __restrict__ T) {
     const int sub = threadIdx.x % LANES;
-    const int texel = (blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+    const int texel = (bx * blockDim.x + threadIdx.x) / LANES;
     const int n = P.t_mu * P.t_r;
     const bool active = texel < n;
     const int i = active ? texel % P.t_mu : 0, j = active ? texel / P.t_mu : 0;
@@ -160,11 +160,11 @@ __global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __re
 // ====================================================== 2. direct irradiance
 // direct_irradiance.comp:10-46
 template <typename G>
-__global__ __launch_bounds__(256) void k_direct_irradiance(DevParams P,
+__device__ __forceinline__ void direct_irradiance_block(const DevParams& P, const int bx, const int by, const int bz,
                                                            const float4* __restrict__ T,
                                                            float4* __restrict__ dE) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
+    int i = bx * blockDim.x + threadIdx.x;
+    int j = by;
     if (i >= P.i_mus || j >= P.i_r) return;
     G x_mu_s = G(i) / G(P.i_mus - 1);
     G x_r = G(j) / G(P.i_r - 1);
@@ -194,7 +194,7 @@ constexpr int kMsSamples = 51;
 // evaluates only its sun transmittance T(r_d, mu_s_d) (one bilinear
 // lookup) and accumulates.
 template <typename G, bool COMP>
-__global__ __launch_bounds__(256) void k_single_scattering(DevParams P,
+__device__ __forceinline__ void single_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                            const float4* __restrict__ T,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_single_scattering(DevParams P,
     __shared__ int2 s_trow[kMsSamples];   // T-LUT rows of r_d
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
-    const int y = blockIdx.x, z = blockIdx.y;
+    const int y = bx, z = by;
     const int tid = threadIdx.x, nt = blockDim.x;
 
     TexelCoord<G> rc = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z));
@@ -437,8 +437,7 @@ inline size_t density_lds_bytes(int nu, int i_mus, bool single) {
 // (phi_m, phi_{31-m}): both share cos(phi), hence nu2 and the two phase
 // functions; their sin(phi) terms have opposite signs.
 template <typename G, bool SINGLE>
-__global__ __launch_bounds__(256, 8) void k_scattering_density(
-    DevParams P, const float4* __restrict__ T, const float4* __restrict__ dR,
+__device__ __forceinline__ void scattering_density_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ T, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J) {
     extern __shared__ float4 s_dyn[];
@@ -450,8 +449,8 @@ __global__ __launch_bounds__(256, 8) void k_scattering_density(
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu, NE = P.i_mus;
     const int W = NU * MUS;
     const int nslots = NU + 2, nent = NU + 1;
-    const int j = blockIdx.y;  // mu_s index within a nu slice
-    const int z = blockIdx.z;  // r index
+    const int j = by;  // mu_s index within a nu slice
+    const int z = bz;  // r index
     const int tid = threadIdx.x;
     float4* s_raw = s_dyn;                                   // [NTEX][16][nslots]
     float4* s_eraw = s_raw + NTEX * kDenTheta * nslots;      // [NE]
@@ -525,7 +524,7 @@ __global__ __launch_bounds__(256, 8) void k_scattering_density(
     }
     __syncthreads();
 
-    const int t = blockIdx.x * blockDim.x + tid;  // texel within the tile
+    const int t = bx * blockDim.x + tid;  // texel within the tile
     if (t >= MU * NU) return;
     const int k = t % NU, y = t / NU;
     const unsigned x = unsigned(k * MUS + j);
@@ -607,14 +606,13 @@ __host__ __device__ inline size_t indirect_lds_bytes_per_wave(int nu, bool singl
 }
 
 template <typename G, bool SINGLE>
-__global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
-    DevParams P, const float4* __restrict__ dR, const float4* __restrict__ dM,
+__device__ __forceinline__ void indirect_irradiance_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ dR, const float4* __restrict__ dM,
     const float4* __restrict__ dMS, float4* __restrict__ dE, float4* __restrict__ E) {
     extern __shared__ float4 s_dyn[];
     constexpr int NTEX = SINGLE ? 2 : 1;
     const int waves = blockDim.x / 64;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int texel = blockIdx.x * waves + wave;
+    const int texel = bx * waves + wave;
     const int n = P.i_mus * P.i_r;
     const bool active = texel < n;
     const int ti = active ? texel % P.i_mus : 0, tj = active ? texel / P.i_mus : 0;
@@ -698,7 +696,7 @@ __global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
 constexpr int kMsTexelsPerThread = 4;
 
 template <typename G, bool COMP>
-__global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
+__device__ __forceinline__ void multiple_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                              const float4* __restrict__ T,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
@@ -709,7 +707,7 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
     __shared__ int4 s_off[kMsSamples];          // offsets of the 4 (z, y) rows
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
-    const int y = blockIdx.x, z = blockIdx.y;
+    const int y = bx, z = by;
     const int tid = threadIdx.x, nt = blockDim.x;
 
     // Row geometry: r and mu depend on (y, z) only.
@@ -832,10 +830,135 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
     }
 }
 
+
+// ================================================== kernel entry points
+// Plain kernels: one atmosphere, block coordinates from blockIdx, the
+// parameters as a kernel argument.  Batched kernels (kb_*): one launch over
+// many atmospheres; each block reads a 32-bit tile word (atmosphere slot in
+// the top 8 bits, block index within that atmosphere's grid in the low 24)
+// and the atmosphere's parameters and LUT pointers from HBM arrays.
+__device__ __forceinline__ void decode_tile(uint32_t t, int& atm, int& local) {
+    atm = int(t >> 24);
+    local = int(t & 0xffffffu);
+}
+
+template <typename G, bool COMP, int LANES>
+__global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T) {
+    transmittance_block<G, COMP, LANES>(P, blockIdx.x, 0, 0, T);
+}
+template <typename G, bool COMP, int LANES>
+__global__ __launch_bounds__(256) void kb_transmittance(const DevParams* __restrict__ Ps,
+                                                        const LutPtrs* __restrict__ L,
+                                                        const uint32_t* __restrict__ tiles) {
+    int atm, local;
+    decode_tile(tiles[blockIdx.x], atm, local);
+    transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, L[atm].T);
+}
+
+template <typename G>
+__global__ __launch_bounds__(256) void k_direct_irradiance(DevParams P, const float4* __restrict__ T,
+                                                           float4* __restrict__ dE) {
+    direct_irradiance_block<G>(P, blockIdx.x, blockIdx.y, 0, T, dE);
+}
+template <typename G>
+__global__ __launch_bounds__(256) void kb_direct_irradiance(const DevParams* __restrict__ Ps,
+                                                            const LutPtrs* __restrict__ L,
+                                                            const uint32_t* __restrict__ tiles) {
+    int atm, local;
+    decode_tile(tiles[blockIdx.x], atm, local);
+    const DevParams& P = Ps[atm];
+    const int gx = (P.i_mus + 63) / 64;
+    direct_irradiance_block<G>(P, local % gx, local / gx, 0, L[atm].T, L[atm].dE);
+}
+
+template <typename G, bool COMP>
+__global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const float4* __restrict__ T,
+                                                           float4* __restrict__ dR,
+                                                           float4* __restrict__ dM,
+                                                           float4* __restrict__ S) {
+    single_scattering_block<G, COMP>(P, blockIdx.x, blockIdx.y, 0, T, dR, dM, S);
+}
+template <typename G, bool COMP>
+__global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
+                                                            const LutPtrs* __restrict__ L,
+                                                            const uint32_t* __restrict__ tiles) {
+    int atm, local;
+    decode_tile(tiles[blockIdx.x], atm, local);
+    const DevParams& P = Ps[atm];
+    const LutPtrs& l = L[atm];
+    single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.dR, l.dM, l.S);
+}
+
+template <typename G, bool SINGLE>
+__global__ __launch_bounds__(256, 8) void k_scattering_density(
+    DevParams P, const float4* __restrict__ T, const float4* __restrict__ dR,
+    const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
+    float4* __restrict__ J) {
+    scattering_density_block<G, SINGLE>(P, blockIdx.x, blockIdx.y, blockIdx.z, T, dR, dM, dMS, dE,
+                                        J);
+}
+template <typename G, bool SINGLE>
+__global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams* __restrict__ Ps,
+                                                                const LutPtrs* __restrict__ L,
+                                                                const uint32_t* __restrict__ tiles) {
+    int atm, local;
+    decode_tile(tiles[blockIdx.x], atm, local);
+    const DevParams& P = Ps[atm];
+    const LutPtrs& l = L[atm];
+    const int gx = (P.s_mu * P.s_nu + 255) / 256;
+    scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
+                                        local / (gx * P.s_mus), l.T, l.dR, l.dM, l.dMS, l.dE, l.J);
+}
+
+template <typename G, bool SINGLE>
+__global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
+    DevParams P, const float4* __restrict__ dR, const float4* __restrict__ dM,
+    const float4* __restrict__ dMS, float4* __restrict__ dE, float4* __restrict__ E) {
+    indirect_irradiance_block<G, SINGLE>(P, blockIdx.x, 0, 0, dR, dM, dMS, dE, E);
+}
+template <typename G, bool SINGLE>
+__global__ __launch_bounds__(64 * kIndWaves) void kb_indirect_irradiance(
+    const DevParams* __restrict__ Ps, const LutPtrs* __restrict__ L,
+    const uint32_t* __restrict__ tiles) {
+    int atm, local;
+    decode_tile(tiles[blockIdx.x], atm, local);
+    const LutPtrs& l = L[atm];
+    indirect_irradiance_block<G, SINGLE>(Ps[atm], local, 0, 0, l.dR, l.dM, l.dMS, l.dE, l.E);
+}
+
+template <typename G, bool COMP>
+__global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
+                                                             const float4* __restrict__ T,
+                                                             const float4* __restrict__ J,
+                                                             float4* __restrict__ dMS,
+                                                             float4* __restrict__ S) {
+    multiple_scattering_block<G, COMP>(P, blockIdx.x, blockIdx.y, 0, T, J, dMS, S);
+}
+template <typename G, bool COMP>
+__global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
+                                                              const LutPtrs* __restrict__ L,
+                                                              const uint32_t* __restrict__ tiles) {
+    int atm, local;
+    decode_tile(tiles[blockIdx.x], atm, local);
+    const DevParams& P = Ps[atm];
+    const LutPtrs& l = L[atm];
+    multiple_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.J, l.dMS, l.S);
+}
+
 // ================================================================== clear
 __global__ void k_fill_zero(float4* __restrict__ p, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+// Clear every atmosphere's irradiance LUT (src/precompute.rs:1802-1816).
+__global__ void kb_clear_irradiance(const DevParams* __restrict__ Ps,
+                                    const LutPtrs* __restrict__ L,
+                                    const uint32_t* __restrict__ tiles) {
+    int atm, local;
+    decode_tile(tiles[blockIdx.x], atm, local);
+    const size_t n = size_t(Ps[atm].i_mus) * Ps[atm].i_r;
+    const size_t i = size_t(local) * blockDim.x + threadIdx.x;
+    if (i < n) L[atm].E[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ========================================================== launch wrappers
@@ -927,6 +1050,93 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* T, const
         k_multiple_scattering<double, true><<<grid, threads, lds, s>>>(P, T, J, dMS, S);
     else
         k_multiple_scattering<float, false><<<grid, threads, lds, s>>>(P, T, J, dMS, S);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ batched
+PassShape batch_pass_shape(const DevParams& P, int pass, int order) {
+    const size_t ni = size_t(P.i_mus) * P.i_r;
+    const bool precise = P.flags & 1;
+    switch (pass) {
+        case kPassTransmittance: {
+            const size_t lanes = precise ? 8 : 1;
+            return {cdiv(size_t(P.t_mu) * P.t_r * lanes, 256), 256, 0};
+        }
+        case kPassDirect:
+            return {cdiv(P.i_mus, 64) * unsigned(P.i_r), 64, 0};
+        case kPassSingle:
+            return {unsigned(P.s_mu) * unsigned(P.s_r), 256, 0};
+        case kPassClear:
+            return {cdiv(ni, 256), 256, 0};
+        case kPassDensity:
+            return {cdiv(size_t(P.s_mu) * P.s_nu, 256) * unsigned(P.s_mus) * unsigned(P.s_r), 256,
+                    density_lds_bytes(P.s_nu, P.i_mus, (order - 1) == 1)};
+        case kPassIndirect:  // one wave per texel, one wave per block
+            return {unsigned(ni), 64, indirect_lds_bytes_per_wave(P.s_nu, (order - 1) == 1)};
+        case kPassMultiple:
+            return {unsigned(P.s_mu) * unsigned(P.s_r), 256,
+                    sizeof(float4) * 2 * size_t(P.s_nu) * P.s_mus};
+    }
+    return {0, 0, 0};
+}
+
+hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
+                        const LutPtrs* L, const uint32_t* tiles, unsigned ntiles,
+                        unsigned threads, size_t lds, hipStream_t s) {
+    if (ntiles == 0) return hipSuccess;
+    const dim3 grid(ntiles), block(threads);
+    const bool single = (order - 1) == 1;
+    switch (pass) {
+        case kPassTransmittance:
+            if (precise)
+                kb_transmittance<double, true, 8><<<grid, block, 0, s>>>(Ps, L, tiles);
+            else
+                kb_transmittance<float, false, 1><<<grid, block, 0, s>>>(Ps, L, tiles);
+            break;
+        case kPassDirect:
+            if (precise)
+                kb_direct_irradiance<double><<<grid, block, 0, s>>>(Ps, L, tiles);
+            else
+                kb_direct_irradiance<float><<<grid, block, 0, s>>>(Ps, L, tiles);
+            break;
+        case kPassSingle:
+            if (precise)
+                kb_single_scattering<double, true><<<grid, block, 0, s>>>(Ps, L, tiles);
+            else
+                kb_single_scattering<float, false><<<grid, block, 0, s>>>(Ps, L, tiles);
+            break;
+        case kPassClear:
+            kb_clear_irradiance<<<grid, block, 0, s>>>(Ps, L, tiles);
+            break;
+        case kPassDensity:
+            if (precise && single)
+                kb_scattering_density<double, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+            else if (precise)
+                kb_scattering_density<double, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+            else if (single)
+                kb_scattering_density<float, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+            else
+                kb_scattering_density<float, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+            break;
+        case kPassIndirect:
+            if (precise && single)
+                kb_indirect_irradiance<double, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+            else if (precise)
+                kb_indirect_irradiance<double, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+            else if (single)
+                kb_indirect_irradiance<float, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+            else
+                kb_indirect_irradiance<float, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+            break;
+        case kPassMultiple:
+            if (precise)
+                kb_multiple_scattering<double, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+            else
+                kb_multiple_scattering<float, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -536,6 +536,7 @@ fb_status fb_pending_relaunch(fb_pending_atmosphere* pa, fb_stream stream) {
     fb_builder* b = pa->builder;
     FB_HIP_TRY(hipSetDevice(b->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (pa->batch) return batch_relaunch(pa, s);
     if (!pa->graph) {
         // Capture the pass schedule once on the builder's private stream.
         std::lock_guard<std::mutex> g(b->mu);
@@ -642,6 +643,7 @@ void fb_pending_destroy(fb_pending_atmosphere* pa) {
     if (!pa) return;
     (void)hipSetDevice(pa->builder->device);
     (void)hipEventSynchronize(pa->done);
+    if (pa->batch) batch_detach(pa);
     if (pa->graph) (void)hipGraphExecDestroy(pa->graph);
     free_pending_scratch(pa);
     (void)hipEventDestroy(pa->done);

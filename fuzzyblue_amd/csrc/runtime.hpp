@@ -114,5 +114,8 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
 fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_a,
                      fb_pending_atmosphere** out_pa);
 void free_pending_scratch(fb_pending_atmosphere* pa);
+// Batches (batch.hip).
+fb_status batch_relaunch(fb_pending_atmosphere* pa, hipStream_t s);
+void batch_detach(fb_pending_atmosphere* pa);
 void free_atmosphere_buffers(fb_atmosphere* a);
 }  // namespace fbrt
