@@ -142,6 +142,49 @@ def measure_render(b, stream, atm, frames=64, w=1920, h=1080, reps=3):
     return frames * px / (ms * 1e-3) / 1e6, ms
 
 
+def measure_batch(b, stream, n=256, seed=4):
+    """The seeded mixed set (workloads.mixed_batch: sizes half/default/double,
+    orders 2..8) as one batch (one launch per pass, graph replay) against the
+    plain pipeline (each atmosphere's own schedule), both timed with HIP
+    events on the same stream."""
+    from fuzzyblue_amd.workloads import mixed_batch
+
+    ps = mixed_batch(n, seed, precise=True)
+    work = sum(costmodel.lut_set_work(p).roof_seconds() for p in ps)
+    pend = Atmosphere.build_batch(b, stream, ps)
+    for pa in pend:
+        pa.wait()
+    t = Timer()
+    reps = 2
+    t.start(stream)
+    for _ in range(reps):
+        pend[0].relaunch(stream)
+    t.stop(stream)
+    ms_batch = t.ms() / reps
+    for pa in pend:
+        pa.close()
+    ms_plain = 0.0
+    for p in ps:
+        pa = Atmosphere.build(b, stream, p)
+        pa.wait()
+        t.start(stream)
+        pa.relaunch(stream)
+        t.stop(stream)
+        ms_plain += t.ms()
+        pa.close()
+    return {
+        "atmospheres": n,
+        "seed": seed,
+        "batched_lut_sets_s": n / (ms_batch * 1e-3),
+        "plain_lut_sets_s": n / (ms_plain * 1e-3),
+        "speedup_vs_plain": ms_plain / ms_batch,
+        "batched_ms": ms_batch,
+        "plain_ms": ms_plain,
+        "roofline_ms": work * 1e3,
+        "roofline_frac": work * 1e3 / ms_batch,
+    }
+
+
 def cpu_baseline(p):
     """FP32 restatement of the reference passes on the host cores, timed on a
     bounded sample: full transmittance / direct / indirect passes, and texel
@@ -209,6 +252,7 @@ def main():
     ap.add_argument("--latency-requests", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE) and exit")
@@ -266,6 +310,12 @@ def main():
         sky = {"mpixel_s": mpx, "ms_per_64_frames": rms, "frames": 64, "width": 1920,
                "height": 1080}
 
+    batch = None
+    if not args.no_batch:
+        log("[bench] mixed batch of 256 atmospheres")
+        batch = measure_batch(b, stream)
+        log(f"[bench] batch: {batch}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -300,6 +350,7 @@ def main():
         "max_rel_err_vs_fp64": err,
         "fp32_mode_lut_sets_s": args.steps / (ms32 * 1e-3),
         "sky": sky,
+        "batch_mixed256": batch,
         "roofline": {
             "bound": "valu",
             "kernel": "scattering_density",

@@ -135,6 +135,7 @@ class fb_server_config(C.Structure):
         ("hbm_budget_bytes", C.c_size_t),
         ("max_inflight", C.c_uint32),
         ("n_streams", C.c_uint32),
+        ("keep_outputs", C.c_uint32),
     ]
 
 

@@ -252,6 +252,31 @@ __global__ __launch_bounds__(256) void k_sun_sky_irradiance(
                                 P.solar[2] * ts.z * ns, 0.0f);
 }
 
+// One synthetic frame with scalar camera altitude / sun zenith (the serving
+// loop's render request; same rays as frame 0 of k_render_frames).
+__global__ __launch_bounds__(256) void k_render_frame1(DevParams P, const float4* __restrict__ T,
+                                                       const float4* __restrict__ S, float alt,
+                                                       float zs, uint32_t pixels, uint64_t seed,
+                                                       float4* __restrict__ rad_out) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    float3 cam = make_float3(0.0f, 0.0f, P.bottom + alt);
+    float3 sun = make_float3(sinf(zs), 0.0f, cosf(zs));
+    float d[3];
+    ray_direction(seed, 0u, p, d);
+    float3 rad, tr;
+    sky_radiance(P, T, S, cam, make_float3(d[0], d[1], d[2]), sun, rad, tr);
+    rad_out[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+}
+
+hipError_t launch_render_frame(const DevParams& P, const float4* T, const float4* S, float alt,
+                               float zs, uint32_t pixels, uint64_t seed, float4* rad,
+                               hipStream_t s) {
+    if (pixels == 0) return hipSuccess;
+    k_render_frame1<<<(pixels + 255) / 256, 256, 0, s>>>(P, T, S, alt, zs, pixels, seed, rad);
+    return hipGetLastError();
+}
+
 }  // namespace fbk
 
 using namespace fbrt;

@@ -1,0 +1,504 @@
+// server.hip — a request loop that serves precompute and sky-render
+// requests as they arrive (no reference counterpart: the reference records
+// one precompute into a caller's command buffer and never schedules).
+//
+// Continuous batching: one worker thread owns every HIP call.  Each turn it
+//   1. retires finished work (HIP events / pending queries): a finished
+//      precompute becomes a resident LUT set, a finished render unpins its
+//      LUT set and releases (or keeps) its output;
+//   2. admits waiting renders whose LUT set is resident or already being
+//      built (the render's stream waits on the precompute's event);
+//   3. admits waiting precomputes, in arrival order, while precompute slots
+//      and the HBM budget allow — evicting least-recently-used resident LUT
+//      sets that no render pins — and launches every admitted group of the
+//      same precision as ONE batch (one launch per pass over all of them).
+// Work is spread over several device streams, so renders run beside long
+// precomputes.  Memory admission counts resident LUT sets, in-flight scratch
+// and render outputs against the budget.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace fbk {
+hipError_t launch_render_frame(const DevParams& P, const float4* T, const float4* S, float alt,
+                               float zs, uint32_t pixels, uint64_t seed, float4* rad,
+                               hipStream_t s);
+}
+
+using namespace fbrt;
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+    uint64_t id = 0;
+    uint32_t kind = 0;
+    fb_parameters params{};
+    uint64_t lut_id = 0;
+    float alt = 0.0f, zen = 0.0f;
+    uint32_t w = 0, h = 0;
+    uint64_t seed = 0;
+    double t_submit = 0.0, t_start = 0.0;
+    // in flight
+    fb_pending_atmosphere* pending = nullptr;
+    hipEvent_t ev = nullptr;
+    float4* out = nullptr;
+    size_t out_bytes = 0;
+    size_t reserved = 0;
+};
+
+struct Resident {
+    fb_atmosphere* atm = nullptr;
+    size_t bytes = 0;
+    uint64_t last_use = 0;
+};
+
+}  // namespace
+
+struct fb_server {
+    fb_builder* b = nullptr;
+    fb_server_config cfg{};
+    std::vector<hipStream_t> streams;
+    size_t next_stream = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_idle;
+    bool stop = false;
+    std::deque<Req*> waiting;
+    std::vector<Req*> inflight;
+    std::vector<fb_request_info> finished;
+    std::unordered_map<uint64_t, Resident> resident;
+    std::unordered_map<uint64_t, Req*> building;  // lut_id -> in-flight precompute
+    std::unordered_map<uint64_t, int> pins;       // lut_id -> renders in flight
+    std::unordered_map<uint64_t, std::pair<float4*, size_t>> outputs;
+    size_t in_use = 0, peak = 0;
+    uint64_t next_id = 1, next_lut = 1, clock = 0;
+    fb_server_stats stats{};
+    std::thread worker;
+
+    hipStream_t stream() { return streams[next_stream++ % streams.size()]; }
+
+    void account(size_t add, size_t sub) {
+        in_use = in_use + add - sub;
+        peak = std::max(peak, in_use);
+    }
+
+    void finish(Req* r, fb_status st) {
+        fb_request_info info{};
+        info.id = r->id;
+        info.kind = r->kind;
+        info.status = int32_t(st);
+        info.lut_id = r->lut_id;
+        info.t_submit = r->t_submit;
+        info.t_start = r->t_start;
+        info.t_done = now_s();
+        finished.push_back(info);
+        if (st == FB_OK)
+            stats.completed++;
+        else
+            stats.rejected++;
+        delete r;
+    }
+
+    // Evict LRU resident sets (not pinned, not `keep`) until `need` fits.
+    bool make_room(size_t need, uint64_t keep) {
+        while (in_use + need > cfg.hbm_budget_bytes) {
+            uint64_t victim = 0, oldest = UINT64_MAX;
+            for (auto& kv : resident) {
+                if (kv.first == keep || pins[kv.first] > 0) continue;
+                if (kv.second.last_use < oldest) {
+                    oldest = kv.second.last_use;
+                    victim = kv.first;
+                }
+            }
+            if (!victim) return false;
+            Resident res = resident[victim];
+            resident.erase(victim);
+            pins.erase(victim);
+            fb_atmosphere_destroy(res.atm);
+            account(0, res.bytes);
+            stats.evictions++;
+        }
+        return true;
+    }
+
+    void retire() {
+        std::vector<Req*> still;
+        for (Req* r : inflight) {
+            if (r->kind == FB_REQUEST_PRECOMPUTE) {
+                fb_status st = fb_pending_query(r->pending);
+                if (st == FB_ENOTREADY) {
+                    still.push_back(r);
+                    continue;
+                }
+                building.erase(r->lut_id);
+                if (st == FB_OK) {
+                    fb_atmosphere* a = nullptr;
+                    st = fb_pending_assert_ready(r->pending, &a);
+                    if (st == FB_OK) {
+                        size_t res = 0;
+                        fb_parameters_footprint(&r->params, &res, nullptr);
+                        resident[r->lut_id] = Resident{a, res, ++clock};
+                        account(res, r->reserved);
+                    }
+                } else {
+                    fb_pending_destroy(r->pending);
+                    account(0, r->reserved);
+                }
+                finish(r, st);
+            } else {
+                hipError_t e = hipEventQuery(r->ev);
+                if (e == hipErrorNotReady) {
+                    (void)hipGetLastError();
+                    still.push_back(r);
+                    continue;
+                }
+                (void)hipEventDestroy(r->ev);
+                pins[r->lut_id]--;
+                if (cfg.keep_outputs && e == hipSuccess) {
+                    outputs[r->id] = {r->out, r->out_bytes};
+                } else {
+                    b->pool->release(r->out, r->out_bytes);
+                    account(0, r->out_bytes);
+                }
+                finish(r, e == hipSuccess ? FB_OK : FB_EHIP);
+            }
+        }
+        inflight.swap(still);
+    }
+
+    void admit_renders() {
+        std::deque<Req*> keep;
+        for (Req* r : waiting) {
+            if (r->kind != FB_REQUEST_RENDER) {
+                keep.push_back(r);
+                continue;
+            }
+            fb_atmosphere* atm = nullptr;
+            auto it = resident.find(r->lut_id);
+            if (it != resident.end()) {
+                atm = it->second.atm;
+                it->second.last_use = ++clock;
+            } else {
+                auto bt = building.find(r->lut_id);
+                if (bt != building.end()) {
+                    atm = const_cast<fb_atmosphere*>(fb_pending_peek(bt->second->pending));
+                } else {
+                    bool queued = false;  // its precompute has not been admitted yet
+                    for (Req* q : waiting)
+                        if (q->kind == FB_REQUEST_PRECOMPUTE && q->lut_id == r->lut_id) queued = true;
+                    if (queued) {
+                        keep.push_back(r);
+                        continue;
+                    }
+                    finish(r, FB_ENOENT);
+                    continue;
+                }
+            }
+            size_t bytes = size_t(r->w) * r->h * sizeof(float4);
+            if (in_use + bytes > cfg.hbm_budget_bytes && !make_room(bytes, r->lut_id)) {
+                keep.push_back(r);
+                continue;
+            }
+            void* out = nullptr;
+            if (b->pool->alloc(bytes, &out) != FB_OK) {
+                keep.push_back(r);
+                continue;
+            }
+            hipStream_t s = stream();
+            hipError_t e = hipEventCreateWithFlags(&r->ev, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, atm->ready, 0);
+            if (e == hipSuccess)
+                e = fbk::launch_render_frame(atm->dev, atm->T.p, atm->S.p, r->alt, r->zen,
+                                             r->w * r->h, r->seed, static_cast<float4*>(out), s);
+            if (e == hipSuccess) e = hipEventRecord(r->ev, s);
+            if (e == hipSuccess) e = hipEventRecord(atm->last_use, s);
+            if (e != hipSuccess) {
+                (void)hipStreamSynchronize(s);
+                if (r->ev) (void)hipEventDestroy(r->ev);
+                b->pool->release(out, bytes);
+                finish(r, hip_fail(e, "render launch"));
+                continue;
+            }
+            atm->used = true;
+            r->out = static_cast<float4*>(out);
+            r->out_bytes = bytes;
+            r->t_start = now_s();
+            pins[r->lut_id]++;
+            account(bytes, 0);
+            inflight.push_back(r);
+        }
+        waiting.swap(keep);
+    }
+
+    void admit_precomputes() {
+        size_t slots = cfg.max_inflight;
+        for (Req* r : inflight)
+            if (r->kind == FB_REQUEST_PRECOMPUTE && slots) --slots;
+        std::vector<Req*> group[2];
+        std::deque<Req*> keep;
+        bool blocked = false;
+        for (Req* r : waiting) {
+            if (r->kind != FB_REQUEST_PRECOMPUTE || blocked || slots == 0) {
+                keep.push_back(r);
+                continue;
+            }
+            size_t need = fb_parameters_footprint(&r->params, nullptr, nullptr);
+            if (need > cfg.hbm_budget_bytes) {
+                finish(r, FB_EBUDGET);
+                continue;
+            }
+            if (in_use + need > cfg.hbm_budget_bytes && !make_room(need, 0)) {
+                blocked = true;  // keep arrival order: later requests wait too
+                keep.push_back(r);
+                continue;
+            }
+            r->reserved = need;
+            account(need, 0);
+            group[(r->params.flags & FB_FLAG_PRECISE) ? 1 : 0].push_back(r);
+            --slots;
+        }
+        waiting.swap(keep);
+        for (auto& g : group) {
+            if (g.empty()) continue;
+            std::vector<fb_parameters> ps;
+            for (Req* r : g) ps.push_back(r->params);
+            std::vector<fb_pending_atmosphere*> outs(g.size(), nullptr);
+            hipStream_t s = stream();
+            fb_status st = g.size() == 1
+                               ? fb_atmosphere_build(b, s, &ps[0], &outs[0])
+                               : fb_atmosphere_build_batch(b, s, ps.data(), ps.size(), outs.data());
+            double t = now_s();
+            for (size_t i = 0; i < g.size(); ++i) {
+                Req* r = g[i];
+                if (st != FB_OK) {
+                    account(0, r->reserved);
+                    finish(r, st);
+                    continue;
+                }
+                r->pending = outs[i];
+                r->t_start = t;
+                building[r->lut_id] = r;
+                inflight.push_back(r);
+            }
+        }
+    }
+
+    void run() {
+        std::unique_lock<std::mutex> lk(mu);
+        (void)hipSetDevice(b->device);
+        while (true) {
+            if (waiting.empty() && inflight.empty()) {
+                cv_idle.notify_all();
+                if (stop) break;
+                cv_work.wait(lk, [&] { return stop || !waiting.empty(); });
+                continue;
+            }
+            retire();
+            admit_renders();
+            admit_precomputes();
+            if (inflight.empty() && !waiting.empty()) {
+                // Nothing runs, so nothing will free memory: what is still
+                // waiting cannot fit the budget.
+                for (Req* r : waiting) {
+                    if (r->reserved) account(0, r->reserved);
+                    finish(r, FB_EBUDGET);
+                }
+                waiting.clear();
+            }
+            if (!inflight.empty()) {
+                // Poll completions; new submissions wake the loop early.
+                cv_work.wait_for(lk, std::chrono::microseconds(50));
+            }
+        }
+    }
+};
+
+extern "C" {
+
+fb_status fb_server_create(fb_builder* b, const fb_server_config* cfg, fb_server** out) {
+    if (!b || !cfg || !out) {
+        set_error("NULL argument");
+        return FB_EINVAL;
+    }
+    *out = nullptr;
+    if (cfg->hbm_budget_bytes == 0 || cfg->max_inflight == 0) {
+        set_error("budget and max_inflight must be positive");
+        return FB_EINVAL;
+    }
+    auto* s = new fb_server();
+    s->b = b;
+    builder_retain(b);
+    s->cfg = *cfg;
+    uint32_t ns = cfg->n_streams ? std::min(cfg->n_streams, 16u) : 4u;
+    hipError_t e = hipSetDevice(b->device);
+    for (uint32_t i = 0; i < ns && e == hipSuccess; ++i) {
+        hipStream_t st = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) s->streams.push_back(st);
+    }
+    if (e != hipSuccess) {
+        fb_status st = hip_fail(e, "server streams");
+        for (auto st2 : s->streams) (void)hipStreamDestroy(st2);
+        builder_release(b);
+        delete s;
+        return st;
+    }
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return FB_OK;
+}
+
+void fb_server_destroy(fb_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->worker.join();  // the loop drains every submitted request first
+    (void)hipSetDevice(s->b->device);
+    for (auto& kv : s->resident) fb_atmosphere_destroy(kv.second.atm);
+    for (auto& kv : s->outputs) s->b->pool->release(kv.second.first, kv.second.second);
+    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    builder_release(s->b);
+    delete s;
+}
+
+fb_status fb_server_submit_precompute(fb_server* s, const fb_parameters* p, uint64_t* request_id,
+                                      uint64_t* lut_id) {
+    if (!s || !p) {
+        set_error("NULL argument");
+        return FB_EINVAL;
+    }
+    fb_status st = fb_parameters_validate(p);
+    if (st != FB_OK) return st;
+    auto* r = new Req();
+    r->kind = FB_REQUEST_PRECOMPUTE;
+    r->params = *p;
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        r->id = s->next_id++;
+        r->lut_id = s->next_lut++;
+        s->waiting.push_back(r);
+        s->stats.submitted++;
+        if (request_id) *request_id = r->id;
+        if (lut_id) *lut_id = r->lut_id;
+    }
+    s->cv_work.notify_all();
+    return FB_OK;
+}
+
+fb_status fb_server_submit_render(fb_server* s, uint64_t lut_id, float altitude_km,
+                                  float sun_zenith_rad, uint32_t width, uint32_t height,
+                                  uint64_t seed, uint64_t* request_id) {
+    if (!s || width == 0 || height == 0) {
+        set_error("invalid render request");
+        return FB_EINVAL;
+    }
+    auto* r = new Req();
+    r->kind = FB_REQUEST_RENDER;
+    r->lut_id = lut_id;
+    r->alt = altitude_km;
+    r->zen = sun_zenith_rad;
+    r->w = width;
+    r->h = height;
+    r->seed = seed;
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        r->id = s->next_id++;
+        s->waiting.push_back(r);
+        s->stats.submitted++;
+        if (request_id) *request_id = r->id;
+    }
+    s->cv_work.notify_all();
+    return FB_OK;
+}
+
+fb_status fb_server_poll(fb_server* s, fb_request_info* out, size_t cap, size_t* n) {
+    if (!s || !n || (!out && cap)) {
+        set_error("NULL argument");
+        return FB_EINVAL;
+    }
+    std::lock_guard<std::mutex> g(s->mu);
+    size_t k = std::min(cap, s->finished.size());
+    for (size_t i = 0; i < k; ++i) out[i] = s->finished[i];
+    s->finished.erase(s->finished.begin(), s->finished.begin() + k);
+    *n = k;
+    return FB_OK;
+}
+
+fb_status fb_server_drain(fb_server* s) {
+    if (!s) return FB_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_work.notify_all();
+    s->cv_idle.wait(lk, [&] { return s->waiting.empty() && s->inflight.empty(); });
+    return FB_OK;
+}
+
+fb_status fb_server_download(fb_server* s, uint64_t lut_id, fb_lut which, void* host,
+                             size_t bytes) {
+    if (!s) return FB_EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->resident.find(lut_id);
+    if (it == s->resident.end()) {
+        set_error("LUT set %llu is not resident", (unsigned long long)lut_id);
+        return FB_ENOENT;
+    }
+    it->second.last_use = ++s->clock;
+    return fb_atmosphere_download(it->second.atm, which, FB_FORMAT_F32, host, bytes);
+}
+
+fb_status fb_server_render_result(fb_server* s, uint64_t request_id, void* host, size_t bytes) {
+    if (!s || !host) return FB_EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->outputs.find(request_id);
+    if (it == s->outputs.end()) {
+        set_error("no kept output for request %llu", (unsigned long long)request_id);
+        return FB_ENOENT;
+    }
+    if (bytes < it->second.second) {
+        set_error("host buffer too small");
+        return FB_EINVAL;
+    }
+    FB_HIP_TRY(hipMemcpy(host, it->second.first, it->second.second, hipMemcpyDeviceToHost));
+    s->b->pool->release(it->second.first, it->second.second);
+    s->account(0, it->second.second);
+    s->outputs.erase(it);
+    return FB_OK;
+}
+
+fb_status fb_server_get_stats(fb_server* s, fb_server_stats* out) {
+    if (!s || !out) return FB_EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    *out = s->stats;
+    out->hbm_in_use = s->in_use;
+    out->hbm_peak = s->peak;
+    out->resident_sets = uint32_t(s->resident.size());
+    return FB_OK;
+}
+
+fb_status fb_server_resident(fb_server* s, uint64_t* ids, size_t cap, size_t* n) {
+    if (!s || !n || (!ids && cap)) return FB_EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    size_t k = 0;
+    for (auto& kv : s->resident) {
+        if (k < cap) ids[k] = kv.first;
+        ++k;
+    }
+    *n = std::min(k, cap);
+    return FB_OK;
+}
+
+}  // extern "C"

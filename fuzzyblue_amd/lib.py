@@ -103,6 +103,7 @@ def load():
         "fb_server_download": (S, [V, C.c_uint64, S, V, C.c_size_t]),
         "fb_server_render_result": (S, [V, C.c_uint64, V, C.c_size_t]),
         "fb_server_get_stats": (S, [V, C.POINTER(abi.fb_server_stats)]),
+        "fb_server_resident": (S, [V, V, C.c_size_t, C.POINTER(C.c_size_t)]),
         "fb_device_count": (S, [C.POINTER(C.c_int)]),
         "fb_device_alloc": (S, [C.c_size_t, C.POINTER(V)]),
         "fb_device_free": (S, [V]),

@@ -335,9 +335,12 @@ fb_status fb_sun_and_sky_irradiance(fb_builder *b, fb_stream stream,
 typedef struct fb_server fb_server;
 
 typedef struct fb_server_config {
-    size_t hbm_budget_bytes;   /* resident LUT sets + in-flight scratch */
+    size_t hbm_budget_bytes;   /* resident LUT sets + in-flight scratch and
+                                  render outputs */
     uint32_t max_inflight;     /* precompute slots on the device */
     uint32_t n_streams;        /* device streams the server schedules on */
+    uint32_t keep_outputs;     /* nonzero: keep each render's radiance until
+                                  fb_server_render_result fetches it */
 } fb_server_config;
 
 typedef enum fb_request_kind {
@@ -384,6 +387,9 @@ typedef struct fb_server_stats {
     uint32_t resident_sets;
 } fb_server_stats;
 fb_status fb_server_get_stats(fb_server *s, fb_server_stats *out);
+/* Ids of the LUT sets currently resident (finished and not evicted). */
+fb_status fb_server_resident(fb_server *s, uint64_t *ids, size_t cap,
+                             size_t *n);
 
 /* ---------------------------------------------------------------------- */
 /* Device plumbing for callers without their own HIP bindings (tests).     */

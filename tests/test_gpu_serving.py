@@ -1,0 +1,112 @@
+"""The serving loop (fb_server_*): continuous batching of precompute and
+render requests, HBM-budget admission with LRU eviction, and the
+north-star check that each result is the same alone and in the mix."""
+import numpy as np
+import pytest
+
+from fuzzyblue_amd import Parameters, abi
+from fuzzyblue_amd.params import DUMP_SIZES, RESOLUTIONS, SMOKE_SIZES
+from fuzzyblue_amd.serving import Server
+from fuzzyblue_amd.workloads import random_atmosphere
+
+pytestmark = pytest.mark.gpu
+
+GB = 1 << 30
+
+
+def shapes(p):
+    tw, th = p.transmittance_extent()
+    iw, ih = p.irradiance_extent()
+    sw, sh, sd = p.scattering_extent()
+    return {abi.FB_LUT_TRANSMITTANCE: (th, tw, 4), abi.FB_LUT_IRRADIANCE: (ih, iw, 4),
+            abi.FB_LUT_SCATTERING: (sd, sh, sw, 4)}
+
+
+def fetch(server, lut, p):
+    return {k: server.download(lut, k, s) for k, s in shapes(p).items()}
+
+
+def test_alone_vs_mix(builder):
+    target = Parameters().replace(**DUMP_SIZES, order=4, flags=abi.FB_FLAG_PRECISE)
+    # alone
+    s = Server(builder, 8 * GB, keep_outputs=True)
+    _, lut = s.submit_precompute(target)
+    s.drain()
+    alone = fetch(s, lut, target)
+    rid = s.submit_render(lut, 3.0, 0.7, 96, 64, 12345)
+    s.drain()
+    alone_img = s.render_result(rid, 96, 64)
+    infos = s.poll()
+    assert all(i.status == abi.FB_OK for i in infos)
+    s.close()
+    # in a mix: other precomputes (batched with the target) and renders
+    rng = np.random.default_rng(5)
+    s = Server(builder, 8 * GB, keep_outputs=True)
+    others = [random_atmosphere(rng, True, resolution="half") for _ in range(3)]
+    s.submit_precompute(others[0])
+    _, lut = s.submit_precompute(target)
+    for p in others[1:]:
+        s.submit_precompute(p)
+    rid = s.submit_render(lut, 3.0, 0.7, 96, 64, 12345)
+    for k in range(6):
+        s.submit_render(lut, float(k), 0.3 * k, 64, 32, k)
+    s.drain()
+    mixed = fetch(s, lut, target)
+    mixed_img = s.render_result(rid, 96, 64)
+    infos = s.poll()
+    assert len(infos) == 11 and all(i.status == abi.FB_OK for i in infos)
+    for k in alone:
+        assert np.array_equal(alone[k], mixed[k]), k
+    assert np.array_equal(alone_img, mixed_img)
+    s.close()
+
+
+def test_budget_rejection_and_unknown_lut(builder):
+    p = Parameters().replace(**SMOKE_SIZES)
+    need = int(p.scattering_extent()[0] * p.scattering_extent()[1] * p.scattering_extent()[2]
+               * 16 * 6)
+    s = Server(builder, need // 2)
+    rid, _ = s.submit_precompute(p)
+    bad = s.submit_render(987654, 1.0, 0.5, 8, 8, 1)
+    s.drain()
+    st = {i.id: i.status for i in s.poll()}
+    assert st[rid] == abi.FB_EBUDGET
+    assert st[bad] == abi.FB_ENOENT
+    s.close()
+
+
+def test_lru_eviction(builder):
+    p = Parameters().replace(**RESOLUTIONS["half"])
+    res = (p.transmittance_extent()[0] * p.transmittance_extent()[1]
+           + p.irradiance_extent()[0] * p.irradiance_extent()[1]
+           + p.scattering_extent()[0] * p.scattering_extent()[1] * p.scattering_extent()[2]) * 16
+    scratch = (p.irradiance_extent()[0] * p.irradiance_extent()[1]
+               + 4 * p.scattering_extent()[0] * p.scattering_extent()[1]
+               * p.scattering_extent()[2]) * 16
+    # room for two resident sets plus one build in flight
+    s = Server(builder, 2 * res + res + scratch + 4096, max_inflight=1)
+    luts = []
+    for k in range(4):
+        _, lut = s.submit_precompute(p.replace(order=2 + k))
+        luts.append(lut)
+        s.drain()
+    infos = s.poll()
+    assert all(i.status == abi.FB_OK for i in infos)
+    st = s.stats()
+    assert st.evictions >= 1
+    resident = set(s.resident())
+    assert luts[-1] in resident and luts[0] not in resident
+    s.close()
+
+
+def test_render_waits_for_inflight_precompute(builder):
+    p = Parameters().replace(**DUMP_SIZES)
+    s = Server(builder, 8 * GB, keep_outputs=True)
+    _, lut = s.submit_precompute(p)
+    rid = s.submit_render(lut, 1.0, 0.2, 32, 16, 3)
+    s.drain()
+    infos = {i.id: i for i in s.poll()}
+    assert infos[rid].status == abi.FB_OK
+    img = s.render_result(rid, 32, 16)
+    assert np.all(np.isfinite(img)) and img[..., :3].max() > 0.0
+    s.close()
