@@ -185,6 +185,25 @@ def measure_batch(b, stream, n=256, seed=4):
     }
 
 
+def measure_serving(b, rates, duration, budget_gb=200.0):
+    """Poisson arrivals (workloads.arrivals: 20% precompute drawn like the
+    mixed set, 80% sky renders of 640x360 / 1920x1080 / 3840x2160 rays
+    against a random resident LUT set) replayed in real time into the
+    native serving loop with an HBM budget; p50/p99 per request type."""
+    from fuzzyblue_amd.serving import Server, replay
+    from fuzzyblue_amd.workloads import arrivals
+
+    out = {}
+    for rate in rates:
+        reqs = arrivals(rate, duration, seed=5 + int(rate))
+        srv = Server(b, int(budget_gb * 1e9), max_inflight=16, n_streams=4)
+        res = replay(srv, reqs, seed=int(rate))
+        srv.close()
+        out[f"lambda_{rate:g}"] = res
+        log(f"[bench] serving lambda={rate}: {res}")
+    return {"duration_s": duration, "hbm_budget_gb": budget_gb, "runs": out}
+
+
 def cpu_baseline(p):
     """FP32 restatement of the reference passes on the host cores, timed on a
     bounded sample: full transmittance / direct / indirect passes, and texel
@@ -253,6 +272,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serving-rates", default="50",
+                    help="comma-separated Poisson rates (req/s) for the serving run; "
+                         "empty to skip")
+    ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE) and exit")
@@ -316,6 +339,12 @@ def main():
         batch = measure_batch(b, stream)
         log(f"[bench] batch: {batch}")
 
+    serving = None
+    if args.serving_rates:
+        rates = [float(x) for x in args.serving_rates.split(",") if x]
+        log(f"[bench] serving {rates} req/s for {args.serving_seconds} s each")
+        serving = measure_serving(b, rates, args.serving_seconds)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -351,6 +380,7 @@ def main():
         "fp32_mode_lut_sets_s": args.steps / (ms32 * 1e-3),
         "sky": sky,
         "batch_mixed256": batch,
+        "serving": serving,
         "roofline": {
             "bound": "valu",
             "kernel": "scattering_density",

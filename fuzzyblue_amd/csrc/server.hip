@@ -6,14 +6,14 @@
 //   1. retires finished work (HIP events / pending queries): a finished
 //      precompute becomes a resident LUT set, a finished render unpins its
 //      LUT set and releases (or keeps) its output;
-//   2. admits waiting renders whose LUT set is resident or already being
-//      built (the render's stream waits on the precompute's event);
+//   2. admits waiting renders whose LUT set is resident (a render of a set
+//      still being built waits in the queue until the set is resident);
 //   3. admits waiting precomputes, in arrival order, while precompute slots
 //      and the HBM budget allow — evicting least-recently-used resident LUT
 //      sets that no render pins — and launches every admitted group of the
 //      same precision as ONE batch (one launch per pass over all of them).
-// Work is spread over several device streams, so renders run beside long
-// precomputes.  Memory admission counts resident LUT sets, in-flight scratch
+// Renders have a device stream of their own and precomputes rotate over the
+// others, so renders run beside long precomputes instead of behind them.  Memory admission counts resident LUT sets, in-flight scratch
 // and render outputs against the budget.
 #include <algorithm>
 #include <chrono>
@@ -85,7 +85,13 @@ struct fb_server {
     fb_server_stats stats{};
     std::thread worker;
 
-    hipStream_t stream() { return streams[next_stream++ % streams.size()]; }
+    // Stream 0 carries renders only, so a render is never queued behind a
+    // long precompute; precomputes rotate over the other streams.
+    hipStream_t render_stream() { return streams[0]; }
+    hipStream_t precompute_stream() {
+        if (streams.size() == 1) return streams[0];
+        return streams[1 + (next_stream++ % (streams.size() - 1))];
+    }
 
     void account(size_t add, size_t sub) {
         in_use = in_use + add - sub;
@@ -189,9 +195,12 @@ struct fb_server {
                 atm = it->second.atm;
                 it->second.last_use = ++clock;
             } else {
-                auto bt = building.find(r->lut_id);
-                if (bt != building.end()) {
-                    atm = const_cast<fb_atmosphere*>(fb_pending_peek(bt->second->pending));
+                if (building.count(r->lut_id)) {
+                    // Deferred until the set is resident: a stream wait on the
+                    // precompute would stall every later render on the render
+                    // stream behind it.
+                    keep.push_back(r);
+                    continue;
                 } else {
                     bool queued = false;  // its precompute has not been admitted yet
                     for (Req* q : waiting)
@@ -214,7 +223,7 @@ struct fb_server {
                 keep.push_back(r);
                 continue;
             }
-            hipStream_t s = stream();
+            hipStream_t s = render_stream();
             hipError_t e = hipEventCreateWithFlags(&r->ev, hipEventDisableTiming);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, atm->ready, 0);
             if (e == hipSuccess)
@@ -273,7 +282,7 @@ struct fb_server {
             std::vector<fb_parameters> ps;
             for (Req* r : g) ps.push_back(r->params);
             std::vector<fb_pending_atmosphere*> outs(g.size(), nullptr);
-            hipStream_t s = stream();
+            hipStream_t s = precompute_stream();
             fb_status st = g.size() == 1
                                ? fb_atmosphere_build(b, s, &ps[0], &outs[0])
                                : fb_atmosphere_build_batch(b, s, ps.data(), ps.size(), outs.data());
