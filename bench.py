@@ -43,6 +43,10 @@ from fuzzyblue_amd import (  # noqa: E402
 from fuzzyblue_amd.lib import check, load  # noqa: E402
 
 METRIC = "LUT-sets/s + p50/p99 latency at ≤1e-3 rel err vs FP64, 1×MI355X"
+PROFILE_STEPS = 10
+# rocprofv3 --pmc summary of the density kernel on this workload (FETCH_SIZE
+# and WRITE_SIZE per dispatch, calibrated against a plain copy).
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "scattering_density_default_earth_hbm.txt")
 
 
 def log(msg):
@@ -99,6 +103,20 @@ def measure_latency(b, stream, p, n):
         pa.close()
     lat = np.array(lat)
     return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def density_traffic():
+    """HBM bytes per density launch (average over the order-2 and the two
+    order>=3 launches of one LUT set) from the committed rocprofv3 --pmc
+    summary; None if it is absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split()[1])
+    except OSError:
+        pass
+    return None
 
 
 def max_err_vs_fixture(atm):
@@ -305,8 +323,14 @@ def main():
     ms_per_step = ms / args.steps
     log(f"[bench] {value:.1f} LUT-sets/s ({ms_per_step:.3f} ms/set)")
 
-    # Per-launch profile of the same schedule (HIP events on the launch stream).
-    prof = pa.profile(stream)
+    # Per-launch profile of the same schedule (HIP events on the launch
+    # stream around each launch), over PROFILE_STEPS consecutive steps run
+    # right after the timed region.
+    profs = [pa.profile(stream) for _ in range(PROFILE_STEPS)]
+    prof = [(name, order, t / PROFILE_STEPS) for name, order, t in profs[0]]
+    for other in profs[1:]:
+        prof = [(name, order, t + u / PROFILE_STEPS)
+                for (name, order, t), (_, _, u) in zip(prof, other)]
     passes = {}
     for name, order, t in prof:
         passes.setdefault(name, []).append(t)
@@ -388,9 +412,13 @@ def main():
             "peak": costmodel.PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / costmodel.PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": density_traffic(),
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch, "
+                              "profiles/scattering_density_default_earth_hbm.txt",
             "algorithmic_bytes_per_launch": dens_bytes / max(len(dens), 1),
             "avg_launch_ms": dens_ms / max(len(dens), 1),
+            "launches": len(dens),
+            "profile_steps": PROFILE_STEPS,
         },
         "lut_set_roofline_ms": work.roof_seconds() * 1e3,
         "passes_ms": {k: float(np.sum(v)) for k, v in passes.items()},

@@ -19,8 +19,7 @@ LIB = os.path.join(HERE, "libfuzzyblue_hip.so")
 
 ARCH = os.environ.get("FB_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["precompute.hip", "batch_kernels.hip", "render.hip", "runtime.hip", "batch.hip",
-           "server.hip"]
+SOURCES = ["precompute.hip", "render.hip", "runtime.hip", "batch.hip", "server.hip"]
 
 COMMON = [
     f"--offload-arch={ARCH}",

@@ -292,12 +292,15 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
 //
 // Grid: (ceil(MU*NU/256), MUS, R); each block serves one (r, mu_s) tile.
 template <typename G>
-__device__ __forceinline__ float4 trilinear_slice(const DevParams& P,
-                                                  const float4* __restrict__ tex, int slice,
-                                                  G u_mu_s, const Taps& ty, const Taps& tz) {
-    const int W = P.s_nu * P.s_mus;
+__device__ __forceinline__ Taps slice_taps(const DevParams& P, int slice, G u_mu_s) {
     G x = (G(slice) + u_mu_s) / G(P.s_nu);
-    Taps tx = linear_taps<G>(x, W);
+    return linear_taps<G>(x, P.s_nu * P.s_mus);
+}
+
+__device__ __forceinline__ float4 trilinear_taps(const DevParams& P,
+                                                 const float4* __restrict__ tex, const Taps& tx,
+                                                 const Taps& ty, const Taps& tz) {
+    const int W = P.s_nu * P.s_mus;
     const float4* p00 = tex + (size_t(tz.i0) * P.s_mu + ty.i0) * W;
     const float4* p01 = tex + (size_t(tz.i0) * P.s_mu + ty.i1) * W;
     const float4* p10 = tex + (size_t(tz.i1) * P.s_mu + ty.i0) * W;
@@ -443,6 +446,9 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     extern __shared__ float4 s_dyn[];
     __shared__ float4 s_gv[kDenTheta];    // (T_g*albedo/pi).rgb, distance_to_ground
     __shared__ float s_ginv[kDenTheta];   // 1/|r*zenith + omega_i*d_g|
+    __shared__ Taps s_ty[kDenTheta];      // mu taps of each theta row
+    __shared__ Taps s_tx[kMaxNu + 2];     // x taps of each nu slot -1..NU
+    __shared__ Taps s_tz;                 // r taps of the tile
     __shared__ int s_ground_mask;
 
     constexpr int NTEX = SINGLE ? 2 : 1;
@@ -457,35 +463,19 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     ProfEntry* s_prof = reinterpret_cast<ProfEntry*>(s_eraw + NE);  // [NTEX][16][nent]
     ProfEntry* s_erow = s_prof + NTEX * kDenTheta * nent;    // [NE+1]
 
-    // Tile: r and mu_s shared by every texel (texel x = j, y = 0).
-    TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
-    const G r = tc.r;
-    const G u_mu_s = scattering_u_mu_s<G>(P, tc.mu_s);
-    const Taps tz = linear_taps<G>(scattering_u_r<G>(P, r), P.s_r);
-
-    // 1. Raw profiles: 16 theta rows x (NU+2) nu slots (-1..NU) per texture.
-    for (int e = tid; e < NTEX * kDenTheta * nslots; e += blockDim.x) {
-        int tex_sel = e / (kDenTheta * nslots);
-        int rem = e % (kDenTheta * nslots);
-        int l = rem / nslots, slot = rem % nslots;
-        G ct = den_cos<G>(l);
-        bool g_l = ray_intersects_ground<G>(P, r, ct);
-        Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r, ct, g_l), MU);
-        const float4* tex = SINGLE ? (tex_sel == 0 ? dR : dM) : dMS;
-        s_raw[e] = trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
-    }
-    // delta_irradiance at r = bottom (irradiance.h:20-30 with x_r = 0).
-    {
-        Taps ty = linear_taps<G>(tex_from_unit<G>(G(0), P.i_r), P.i_r);
-        for (int x = tid; x < NE; x += blockDim.x)
-            s_eraw[x] = lerp4(ldg4(dE + ty.i0 * NE + x), ldg4(dE + ty.i1 * NE + x), ty.w);
-    }
-    // Ground terms per theta row (scattering_density.comp:45-60).  The
-    // ground normal's length does not depend on phi.
+    // 0. Tile geometry, once per theta row / nu slot: r and mu_s are shared
+    // by every texel of the tile (texel x = j, y = 0).  Only the threads
+    // that need them evaluate the G-precision maps, and the resulting taps
+    // go through LDS, so no G value stays live across the gather loops.
     if (tid < kDenTheta) {
+        // Ground terms per theta row (scattering_density.comp:45-60).  The
+        // ground normal's length does not depend on phi.
+        const TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
+        const G r = tc.r;
         int l = tid;
         G ct = den_cos<G>(l);
         bool g_l = ray_intersects_ground<G>(P, r, ct);
+        s_ty[l] = linear_taps<G>(scattering_u_mu<G>(P, r, ct, g_l), MU);
         float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float ginv = 0.0f;
         if (g_l) {
@@ -502,6 +492,30 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         s_ginv[l] = ginv;
         unsigned long long b = __ballot(g_l);
         if (l == 0) s_ground_mask = int(b & 0xffffull);
+    } else if (tid >= 64 && tid < 64 + nslots) {
+        // x taps of nu slot k = -1..NU (a second wavefront).
+        const TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
+        const G u_mu_s = scattering_u_mu_s<G>(P, tc.mu_s);
+        s_tx[tid - 64] = slice_taps<G>(P, tid - 65, u_mu_s);
+        if (tid == 64) s_tz = linear_taps<G>(scattering_u_r<G>(P, tc.r), P.s_r);
+    }
+    // delta_irradiance at r = bottom (irradiance.h:20-30 with x_r = 0).
+    {
+        Taps ty = linear_taps<G>(tex_from_unit<G>(G(0), P.i_r), P.i_r);
+        for (int x = tid; x < NE; x += blockDim.x)
+            s_eraw[x] = lerp4(ldg4(dE + ty.i0 * NE + x), ldg4(dE + ty.i1 * NE + x), ty.w);
+    }
+    __syncthreads();
+    // 1. Raw profiles: 16 theta rows x (NU+2) nu slots (-1..NU) per texture.
+    {
+        const Taps tz = s_tz;
+        for (int e = tid; e < NTEX * kDenTheta * nslots; e += blockDim.x) {
+            int tex_sel = e / (kDenTheta * nslots);
+            int rem = e % (kDenTheta * nslots);
+            int l = rem / nslots, slot = rem % nslots;
+            const float4* tex = SINGLE ? (tex_sel == 0 ? dR : dM) : dMS;
+            s_raw[e] = trilinear_taps(P, tex, s_tx[slot], s_ty[l], tz);
+        }
     }
     __syncthreads();
     // 2. (value, delta) entries.
@@ -541,7 +555,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     dt.osx = fminf(fmaxf(float(sdx), -1.0f), 1.0f);
     dt.osy = fminf(fmaxf(float(sdy), -1.0f), 1.0f);
     dt.osz = float(c.mu_s);
-    const float rf = float(r);
+    const float rf = float(c.r);
     const float alt = float(c.r - G(P.bottom));
     const float rho_r = profile_density(P.prof[0], alt);
     const float rho_m = profile_density(P.prof[1], alt);
@@ -634,7 +648,7 @@ __device__ __forceinline__ void indirect_irradiance_block(const DevParams& P, co
         G ct = ind_cos<G>(jj);
         Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r, ct, false), P.s_mu);
         const float4* tex = SINGLE ? (tex_sel == 0 ? dR : dM) : dMS;
-        s_raw[e] = trilinear_slice<G>(P, tex, slot - 1, u_mu_s, ty, tz);
+        s_raw[e] = trilinear_taps(P, tex, slice_taps<G>(P, slot - 1, u_mu_s), ty, tz);
     }
     __syncthreads();
     for (int e = lane; e < NTEX * kIndTheta * nent; e += 64) {
