@@ -202,7 +202,9 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
     __shared__ float4 s_tv[kMsSamples];   // T(r,mu,d_i) * weight (rgb), d_i
     __shared__ float4 s_geo[kMsSamples];  // r_d, rho_rayleigh(r_d), rho_mie(r_d), sin_theta_h
     __shared__ float4 s_lut[kMsSamples];  // cos_theta_h, d_min, d_max - d_min, T-LUT row weight
+    __shared__ float4 s_inv[kMsSamples];  // 1/r_d, 1/(d_max - d_min), top^2 - r_d^2 (precise)
     __shared__ int2 s_trow[kMsSamples];   // T-LUT rows of r_d
+    constexpr bool kPrecise = sizeof(G) == sizeof(double);
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
     const int y = bx, z = by;
@@ -230,12 +232,15 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
         s_geo[i] = make_float4(float(r_d), profile_density(P.prof[0], alt),
                                profile_density(P.prof[1], alt), float(sin_h));
         s_lut[i] = make_float4(float(cos_h), float(d_min), float(d_max - d_min), tr.w);
+        s_inv[i] = make_float4(float(G(1) / r_d), float(G(1) / (d_max - d_min)),
+                               float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))), 0.0f);
         s_trow[i] = make_int2(tr.i0 * P.t_mu, tr.i1 * P.t_mu);
     }
     __syncthreads();
     const float rf = float(r);
     const float alpha = P.sun_angular_radius;
     const float top2 = P.top * P.top;
+    const float tmax = float(P.t_mu - 1);
     for (int x = tid; x < W; x += nt) {
         TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
         const float mu_s = float(c.mu_s), nu = float(c.nu);
@@ -244,12 +249,32 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
             const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i];
             const int2 rows = s_trow[i];
             const float r_d = gq.x, d_i = tv.w;
-            const float mu_s_d = clamp_cosine<float>((rf * mu_s + d_i * nu) / r_d);
-            // Sun transmittance T(r_d, mu_s_d) (transmittance.h:26-33, 63-74).
-            float disc = r_d * r_d * (mu_s_d * mu_s_d - 1.0f) + top2;
-            float d = clamp_distance<float>(-r_d * mu_s_d + safe_sqrt<float>(disc));
-            float x_mu = (d - lq.y) / lq.z;
-            Taps tx = linear_taps<float>(tex_from_unit<float>(x_mu, P.t_mu), P.t_mu);
+            Taps tx;
+            float mu_s_d;
+            if constexpr (kPrecise) {
+                // Sun-ray distance to the top boundary without cancellation:
+                // for mu_s_d > 0 the root is (top^2 - r_d^2) / (r_d mu_s_d +
+                // sqrt(disc)), with top^2 - r_d^2 from the FP64 set-up.
+                // Reciprocals of the per-sample divisors come from FP64 too.
+                const float4 iv = s_inv[i];
+                mu_s_d = __builtin_amdgcn_fmed3f(fmaf(rf, mu_s, d_i * nu) * iv.x, -1.0f, 1.0f);
+                const float rm = r_d * mu_s_d;
+                const float sq = __builtin_amdgcn_sqrtf(fmaf(rm, rm, iv.z));
+                const float d = mu_s_d > 0.0f ? iv.z * __builtin_amdgcn_rcpf(rm + sq) : sq - rm;
+                // tex_from_unit then linear_taps: texel coordinate x_mu*(N-1).
+                const float t = __builtin_amdgcn_fmed3f((d - lq.y) * iv.y * tmax, 0.0f, tmax);
+                const float fl = floorf(t);
+                tx.i0 = int(fl);
+                tx.i1 = min(tx.i0 + 1, P.t_mu - 1);
+                tx.w = t - fl;
+            } else {
+                mu_s_d = clamp_cosine<float>((rf * mu_s + d_i * nu) / r_d);
+                // Sun transmittance T(r_d, mu_s_d) (transmittance.h:26-33, 63-74).
+                float disc = r_d * r_d * (mu_s_d * mu_s_d - 1.0f) + top2;
+                float d = clamp_distance<float>(-r_d * mu_s_d + safe_sqrt<float>(disc));
+                float x_mu = (d - lq.y) / lq.z;
+                tx = linear_taps<float>(tex_from_unit<float>(x_mu, P.t_mu), P.t_mu);
+            }
             float4 ts = lerp4(lerp4(ldg4(T + rows.x + tx.i0), ldg4(T + rows.x + tx.i1), tx.w),
                               lerp4(ldg4(T + rows.y + tx.i0), ldg4(T + rows.y + tx.i1), tx.w),
                               lq.w);
@@ -709,7 +734,7 @@ __device__ __forceinline__ void indirect_irradiance_block(const DevParams& P, co
 // each texel then lerps 2 nu slices x 2 mu_s taps from LDS.
 constexpr int kMsTexelsPerThread = 4;
 
-template <typename G, bool COMP>
+template <typename G, bool COMP, int TPT>
 __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                              const float4* __restrict__ T,
                                                              const float4* __restrict__ J,
@@ -745,12 +770,11 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
     }
 
     // Per-texel constants: mu_s, nu, nu slice and its lerp weight.
-    float tmu_s[kMsTexelsPerThread], tnu[kMsTexelsPerThread], tslice[kMsTexelsPerThread],
-        tlerp[kMsTexelsPerThread];
-    Sum<COMP> acc[kMsTexelsPerThread][3];
+    float tmu_s[TPT], tnu[TPT], tslice[TPT], tlerp[TPT];
+    Sum<COMP> acc[TPT][3];
     const float rf = float(r);
 #pragma unroll
-    for (int q = 0; q < kMsTexelsPerThread; ++q) {
+    for (int q = 0; q < TPT; ++q) {
         const int x = tid + q * nt;
         tmu_s[q] = 0.0f;
         tnu[q] = 0.0f;
@@ -800,7 +824,7 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         const float4 tw = s_tw[i];
         const float inv_r_i = s_geo[i].x, d_i = tw.w;
 #pragma unroll
-        for (int q = 0; q < kMsTexelsPerThread; ++q) {
+        for (int q = 0; q < TPT; ++q) {
             const int x = tid + q * nt;
             if (x < W) {
                 // mu_s at the sample (multiple_scattering.comp:38), in [-1, 1].
@@ -814,8 +838,9 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
                 float fl = floorf(xt);
                 float w = xt - fl;
                 int i0 = int(fl);
-                int c0 = min(max(i0, 0), W - 1), c1 = min(max(i0 + 1, 0), W - 1);
-                int c2 = min(max(i0 + MUS, 0), W - 1), c3 = min(max(i0 + MUS + 1, 0), W - 1);
+                // xt >= 0 (xm >= 0, tslice >= 0): only the right edge clamps.
+                int c0 = min(i0, W - 1), c1 = min(i0 + 1, W - 1);
+                int c2 = min(i0 + MUS, W - 1), c3 = min(i0 + MUS + 1, W - 1);
                 float4 p0 = cur[c0], p1 = cur[c1], p2 = cur[c2], p3 = cur[c3];
                 float l = tlerp[q];
                 float v0 = fmaf(w, p1.x - p0.x, p0.x), v1 = fmaf(w, p1.y - p0.y, p0.y),
@@ -830,7 +855,7 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < kMsTexelsPerThread; ++q) {
+    for (int q = 0; q < TPT; ++q) {
         const int x = tid + q * nt;
         if (x < W) {
             const size_t idx = (size_t(z) * MU + y) * W + x;
@@ -940,13 +965,13 @@ __global__ __launch_bounds__(64 * kIndWaves) void kb_indirect_irradiance(
     indirect_irradiance_block<G, SINGLE>(Ps[atm], local, 0, 0, l.dR, l.dM, l.dMS, l.dE, l.E);
 }
 
-template <typename G, bool COMP>
+template <typename G, bool COMP, int TPT>
 __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
                                                              const float4* __restrict__ T,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
                                                              float4* __restrict__ S) {
-    multiple_scattering_block<G, COMP>(P, blockIdx.x, blockIdx.y, 0, T, J, dMS, S);
+    multiple_scattering_block<G, COMP, TPT>(P, blockIdx.x, blockIdx.y, 0, T, J, dMS, S);
 }
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
@@ -956,7 +981,8 @@ __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* _
     decode_tile(tiles[blockIdx.x], atm, local);
     const DevParams& P = Ps[atm];
     const LutPtrs& l = L[atm];
-    multiple_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.J, l.dMS, l.S);
+    multiple_scattering_block<G, COMP, kMsTexelsPerThread>(P, local % P.s_mu, local / P.s_mu, 0, l.T,
+                                                          l.J, l.dMS, l.S);
 }
 
 // ================================================================== clear
@@ -1004,7 +1030,7 @@ hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4*
     threads = threads > 256 ? 256 : threads;
     dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
-        k_single_scattering<double, true><<<grid, threads, 0, s>>>(P, T, dR, dM, S);
+        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S);
     else
         k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S);
     return hipGetLastError();
@@ -1060,10 +1086,16 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* T, const
     if (W > threads * kMsTexelsPerThread) return hipErrorInvalidValue;
     dim3 grid(P.s_mu, P.s_r);
     size_t lds = sizeof(float4) * 2 * size_t(W);
-    if (P.flags & 1)
-        k_multiple_scattering<double, true><<<grid, threads, lds, s>>>(P, T, J, dMS, S);
+    // Texels per thread: 1 up to 256 texels per row, else 2 or 4.  The
+    // arithmetic of a texel does not depend on it.
+    const int tpt = W <= threads ? 1 : (W <= 2 * threads ? 2 : 4);
+    const bool precise = P.flags & 1;
+#define FB_MS(G, TPT) k_multiple_scattering<G, false, TPT><<<grid, threads, lds, s>>>(P, T, J, dMS, S)
+    if (precise)
+        tpt == 1 ? FB_MS(double, 1) : tpt == 2 ? FB_MS(double, 2) : FB_MS(double, 4);
     else
-        k_multiple_scattering<float, false><<<grid, threads, lds, s>>>(P, T, J, dMS, S);
+        tpt == 1 ? FB_MS(float, 1) : tpt == 2 ? FB_MS(float, 2) : FB_MS(float, 4);
+#undef FB_MS
     return hipGetLastError();
 }
 
@@ -1115,7 +1147,7 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
             break;
         case kPassSingle:
             if (precise)
-                kb_single_scattering<double, true><<<grid, block, 0, s>>>(Ps, L, tiles);
+                kb_single_scattering<double, false><<<grid, block, 0, s>>>(Ps, L, tiles);
             else
                 kb_single_scattering<float, false><<<grid, block, 0, s>>>(Ps, L, tiles);
             break;
@@ -1144,7 +1176,7 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
             break;
         case kPassMultiple:
             if (precise)
-                kb_multiple_scattering<double, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_multiple_scattering<double, false><<<grid, block, lds, s>>>(Ps, L, tiles);
             else
                 kb_multiple_scattering<float, false><<<grid, block, lds, s>>>(Ps, L, tiles);
             break;
