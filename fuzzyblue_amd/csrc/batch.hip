@@ -119,8 +119,8 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
         if (st != FB_OK) break;
         plan->members.push_back(pa);
         dps[i] = a->dev;
-        luts[i] = fbk::LutPtrs{a->T.p, a->E.p, a->S.p, pa->dE.p, pa->dR.p, pa->dM.p, pa->dMS.p,
-                               pa->J.p};
+        luts[i] = fbk::LutPtrs{a->T.p,  a->E.p,   a->S.p, pa->dE.p, pa->dR.p,
+                               pa->dM.p, pa->dMS.p, pa->J.p, pa->MR.p};
     }
     auto fail = [&](fb_status code) {
         for (auto* pa : plan->members) fb_pending_destroy(pa);

@@ -20,13 +20,20 @@ constexpr int kIndPhi = 64;
 constexpr int kMaxNu = 32;
 constexpr int kMaxIrrMuS = 256;
 constexpr int kMaxRowTexels = 1024;  // NU * MUS: one scattering row per block
+// Trapezoid samples of the single and multiple scattering integrals:
+// SAMPLE_COUNT + 1 (single_scattering.comp:42, multiple_scattering.comp:21).
+constexpr int kMsSamples = 51;
+// Multiple-scattering row table: per (r, mu) row and sample, the view
+// transmittance x trapezoid weight (rgb) and d_i, then (1/r_i, mu weight, r
+// weight), then the offsets of the four (r, mu) density rows.
+constexpr int kMsRowFloat4s = 3 * kMsSamples;
 
 hipError_t upload_tables();
 
 // Device addresses of one atmosphere's LUTs (batched launches read an HBM
 // array of these, indexed by the tile's atmosphere slot).
 struct LutPtrs {
-    float4 *T, *E, *S, *dE, *dR, *dM, *dMS, *J;
+    float4 *T, *E, *S, *dE, *dR, *dM, *dMS, *J, *MR;
 };
 
 // Passes of a batched schedule (values match fb_pass).
@@ -61,14 +68,14 @@ hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s);
 hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,
                                     hipStream_t s);
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
-                                    float4* dM, float4* S, hipStream_t s);
+                                    float4* dM, float4* S, float4* MR, hipStream_t s);
 hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
                                      const float4* dR, const float4* dM, const float4* dMS,
                                      const float4* dE, float4* J, hipStream_t s);
 hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float4* dR,
                                       const float4* dM, const float4* dMS, float4* dE, float4* E,
                                       hipStream_t s);
-hipError_t launch_multiple_scattering(const DevParams& P, const float4* T, const float4* J,
+hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, const float4* J,
                                       float4* dMS, float4* S, hipStream_t s);
 hipError_t launch_fill_zero(float4* p, size_t n, hipStream_t s);
 

@@ -180,9 +180,6 @@ __device__ __forceinline__ void direct_irradiance_block(const DevParams& P, cons
         make_float4(P.solar[0] * t.x * a, P.solar[1] * t.y * a, P.solar[2] * t.z * a, 0.0f);
 }
 
-// Trapezoid samples of the single and multiple scattering integrals:
-// SAMPLE_COUNT + 1 (single_scattering.comp:42, multiple_scattering.comp:21).
-constexpr int kMsSamples = 51;
 
 // ===================================================== 3. single scattering
 // single_scattering.comp:10-101.
@@ -198,7 +195,8 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
                                                            const float4* __restrict__ T,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
-                                                           float4* __restrict__ S) {
+                                                           float4* __restrict__ S,
+                                                           float4* __restrict__ MR) {
     __shared__ float4 s_tv[kMsSamples];   // T(r,mu,d_i) * weight (rgb), d_i
     __shared__ float4 s_geo[kMsSamples];  // r_d, rho_rayleigh(r_d), rho_mie(r_d), sin_theta_h
     __shared__ float4 s_lut[kMsSamples];  // cos_theta_h, d_min, d_max - d_min, T-LUT row weight
@@ -235,6 +233,21 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
         s_inv[i] = make_float4(float(G(1) / r_d), float(G(1) / (d_max - d_min)),
                                float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))), 0.0f);
         s_trow[i] = make_int2(tr.i0 * P.t_mu, tr.i1 * P.t_mu);
+        // The multiple-scattering passes march the same 51 samples of this
+        // row (multiple_scattering.comp:16-47, same dx and d_i): their
+        // per-sample terms are written once here and read by every order.
+        {
+            G mu_i = clamp_cosine<G>((r * mu + d_i) / r_d);
+            Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r_d, mu_i, ground), MU);
+            Taps tz = linear_taps<G>(scattering_u_r<G>(P, r_d), P.s_r);
+            const float f = float(dx) * w;
+            float4* o = MR + (size_t(z) * MU + y) * kMsRowFloat4s;
+            o[i] = make_float4(tv.x * f, tv.y * f, tv.z * f, float(d_i));
+            o[kMsSamples + i] = make_float4(float(G(1) / r_d), ty.w, tz.w, 0.0f);
+            reinterpret_cast<int4*>(o + 2 * kMsSamples)[i] =
+                make_int4((tz.i0 * MU + ty.i0) * W, (tz.i0 * MU + ty.i1) * W,
+                          (tz.i1 * MU + ty.i0) * W, (tz.i1 * MU + ty.i1) * W);
+        }
     }
     __syncthreads();
     const float rf = float(r);
@@ -736,38 +749,25 @@ constexpr int kMsTexelsPerThread = 4;
 
 template <typename G, bool COMP, int TPT>
 __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
-                                                             const float4* __restrict__ T,
+                                                             const float4* __restrict__ MR,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
                                                              float4* __restrict__ S) {
     extern __shared__ float4 s_row[];           // [2][W]
-    __shared__ float4 s_tw[kMsSamples];         // T(r,mu,d_i)*dx*weight (rgb), d_i
-    __shared__ float4 s_geo[kMsSamples];        // 1/r_i, wy, wz
-    __shared__ int4 s_off[kMsSamples];          // offsets of the 4 (z, y) rows
+    __shared__ float4 s_desc[kMsRowFloat4s];    // the row's entries of the table MR
+    const float4* s_tw = s_desc;                // T(r,mu,d_i)*dx*weight (rgb), d_i
+    const float4* s_geo = s_desc + kMsSamples;  // 1/r_i, wy, wz
+    const int4* s_off = reinterpret_cast<const int4*>(s_desc + 2 * kMsSamples);
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
     const int y = bx, z = by;
     const int tid = threadIdx.x, nt = blockDim.x;
 
-    // Row geometry: r and mu depend on (y, z) only.
-    TexelCoord<G> rc = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z));
-    const G r = rc.r, mu = rc.mu;
-    const bool ground = rc.ground;
-    const G dx = distance_to_nearest<G>(P, r, mu, ground) / G(kMsSamples - 1);
-    if (tid < kMsSamples) {
-        const int i = tid;
-        G d_i = G(i) * dx;
-        G r_i = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r));
-        G mu_i = clamp_cosine<G>((r * mu + d_i) / r_i);
-        Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r_i, mu_i, ground), MU);
-        Taps tz = linear_taps<G>(scattering_u_r<G>(P, r_i), P.s_r);
-        float3 tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
-        float f = float(dx) * ((i == 0 || i == kMsSamples - 1) ? 0.5f : 1.0f);
-        s_tw[i] = make_float4(tv.x * f, tv.y * f, tv.z * f, float(d_i));
-        s_geo[i] = make_float4(float(G(1) / r_i), ty.w, tz.w, 0.0f);
-        s_off[i] = make_int4((tz.i0 * MU + ty.i0) * W, (tz.i0 * MU + ty.i1) * W,
-                             (tz.i1 * MU + ty.i0) * W, (tz.i1 * MU + ty.i1) * W);
-    }
+    // Row geometry: the 51 samples' terms, written by the single-scattering
+    // pass (the same samples, single_scattering.comp:30-65).
+    const float4* desc = MR + (size_t(z) * MU + y) * kMsRowFloat4s;
+    for (int e = tid; e < kMsRowFloat4s; e += nt) s_desc[e] = desc[e];
+    const G r = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z)).r;
 
     // Per-texel constants: mu_s, nu, nu slice and its lerp weight.
     float tmu_s[TPT], tnu[TPT], tslice[TPT], tlerp[TPT];
@@ -802,25 +802,27 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
     const float invA = 1.0f / (-2.0f * P.mu_s_min * P.bottom * inv_range);
     const float musm1 = float(MUS - 1), fMUS = float(MUS);
 
-    auto stage = [&](int i, float4* dst) {
-        const int4 o = s_off[i];
-        const float4 gq = s_geo[i];
-        for (int x = tid; x < W; x += nt) {
-            float4 r00 = ldg4(J + o.x + x), r01 = ldg4(J + o.y + x);
-            float4 r10 = ldg4(J + o.z + x), r11 = ldg4(J + o.w + x);
-            float ax = fmaf(gq.y, r01.x - r00.x, r00.x), ay = fmaf(gq.y, r01.y - r00.y, r00.y),
-                  az = fmaf(gq.y, r01.z - r00.z, r00.z);
-            float bx = fmaf(gq.y, r11.x - r10.x, r10.x), by = fmaf(gq.y, r11.y - r10.y, r10.y),
-                  bz = fmaf(gq.y, r11.z - r10.z, r10.z);
-            dst[x] = make_float4(fmaf(gq.z, bx - ax, ax), fmaf(gq.z, by - ay, ay),
-                                 fmaf(gq.z, bz - az, az), 0.0f);
-        }
+    // The (mu, r)-bilinear density row of sample i, for texel x.
+    struct Rows {
+        float4 r00, r01, r10, r11;
     };
-    stage(0, s_row);
-    __syncthreads();
-    for (int i = 0; i < kMsSamples; ++i) {
-        const float4* cur = s_row + (i & 1) * W;
-        if (i + 1 < kMsSamples) stage(i + 1, s_row + ((i + 1) & 1) * W);
+    auto fetch = [&](int i, int x) -> Rows {
+        const int4 o = s_off[i];
+        return {ldg4(J + o.x + x), ldg4(J + o.y + x), ldg4(J + o.z + x), ldg4(J + o.w + x)};
+    };
+    auto put = [&](int i, const Rows& q, float4* dst, int x) {
+        const float4 gq = s_geo[i];
+        float ax = fmaf(gq.y, q.r01.x - q.r00.x, q.r00.x),
+              ay = fmaf(gq.y, q.r01.y - q.r00.y, q.r00.y),
+              az = fmaf(gq.y, q.r01.z - q.r00.z, q.r00.z);
+        float bx = fmaf(gq.y, q.r11.x - q.r10.x, q.r10.x),
+              by = fmaf(gq.y, q.r11.y - q.r10.y, q.r10.y),
+              bz = fmaf(gq.y, q.r11.z - q.r10.z, q.r10.z);
+        dst[x] = make_float4(fmaf(gq.z, bx - ax, ax), fmaf(gq.z, by - ay, ay),
+                             fmaf(gq.z, bz - az, az), 0.0f);
+    };
+    // Sample i of every texel of this thread, from the staged row `cur`.
+    auto eval = [&](int i, const float4* __restrict__ cur) {
         const float4 tw = s_tw[i];
         const float inv_r_i = s_geo[i].x, d_i = tw.w;
 #pragma unroll
@@ -851,6 +853,30 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
                 acc[q][1].add(fmaf(l, u1 - v1, v1) * tw.y);
                 acc[q][2].add(fmaf(l, u2 - v2, v2) * tw.z);
             }
+        }
+    };
+    for (int x = tid; x < W; x += nt) put(0, fetch(0, x), s_row, x);
+    __syncthreads();
+    for (int i = 0; i < kMsSamples; ++i) {
+        const float4* cur = s_row + (i & 1) * W;
+        float4* nxt = s_row + ((i + 1) & 1) * W;
+        const bool next = i + 1 < kMsSamples;
+        if constexpr (TPT <= 2) {
+            // Issue the loads of sample i+1 first, so that their latency
+            // overlaps the evaluation of sample i.  (Each thread stages
+            // exactly its own texels x = tid + q*nt, since W <= TPT*nt.)
+            Rows pre[TPT];
+#pragma unroll
+            for (int q = 0; q < TPT; ++q)
+                if (next && tid + q * nt < W) pre[q] = fetch(i + 1, tid + q * nt);
+            eval(i, cur);
+#pragma unroll
+            for (int q = 0; q < TPT; ++q)
+                if (next && tid + q * nt < W) put(i + 1, pre[q], nxt, tid + q * nt);
+        } else {
+            if (next)
+                for (int x = tid; x < W; x += nt) put(i + 1, fetch(i + 1, x), nxt, x);
+            eval(i, cur);
         }
         __syncthreads();
     }
@@ -914,8 +940,9 @@ template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const float4* __restrict__ T,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
-                                                           float4* __restrict__ S) {
-    single_scattering_block<G, COMP>(P, blockIdx.x, blockIdx.y, 0, T, dR, dM, S);
+                                                           float4* __restrict__ S,
+                                                           float4* __restrict__ MR) {
+    single_scattering_block<G, COMP>(P, blockIdx.x, blockIdx.y, 0, T, dR, dM, S, MR);
 }
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
@@ -925,7 +952,8 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
     decode_tile(tiles[blockIdx.x], atm, local);
     const DevParams& P = Ps[atm];
     const LutPtrs& l = L[atm];
-    single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.dR, l.dM, l.S);
+    single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.dR, l.dM, l.S,
+                                     l.MR);
 }
 
 template <typename G, bool SINGLE>
@@ -967,11 +995,11 @@ __global__ __launch_bounds__(64 * kIndWaves) void kb_indirect_irradiance(
 
 template <typename G, bool COMP, int TPT>
 __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
-                                                             const float4* __restrict__ T,
+                                                             const float4* __restrict__ MR,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
                                                              float4* __restrict__ S) {
-    multiple_scattering_block<G, COMP, TPT>(P, blockIdx.x, blockIdx.y, 0, T, J, dMS, S);
+    multiple_scattering_block<G, COMP, TPT>(P, blockIdx.x, blockIdx.y, 0, MR, J, dMS, S);
 }
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
@@ -981,8 +1009,8 @@ __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* _
     decode_tile(tiles[blockIdx.x], atm, local);
     const DevParams& P = Ps[atm];
     const LutPtrs& l = L[atm];
-    multiple_scattering_block<G, COMP, kMsTexelsPerThread>(P, local % P.s_mu, local / P.s_mu, 0, l.T,
-                                                          l.J, l.dMS, l.S);
+    multiple_scattering_block<G, COMP, kMsTexelsPerThread>(P, local % P.s_mu, local / P.s_mu, 0,
+                                                          l.MR, l.J, l.dMS, l.S);
 }
 
 // ================================================================== clear
@@ -1024,15 +1052,15 @@ hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4*
 }
 
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
-                                    float4* dM, float4* S, hipStream_t s) {
+                                    float4* dM, float4* S, float4* MR, hipStream_t s) {
     const int W = P.s_nu * P.s_mus;
     int threads = ((W + 63) / 64) * 64;
     threads = threads > 256 ? 256 : threads;
     dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
-        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S);
+        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR);
     else
-        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S);
+        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR);
     return hipGetLastError();
 }
 
@@ -1078,7 +1106,7 @@ hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float
     return hipGetLastError();
 }
 
-hipError_t launch_multiple_scattering(const DevParams& P, const float4* T, const float4* J,
+hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, const float4* J,
                                       float4* dMS, float4* S, hipStream_t s) {
     const int W = P.s_nu * P.s_mus;
     int threads = ((W + 63) / 64) * 64;
@@ -1090,7 +1118,7 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* T, const
     // arithmetic of a texel does not depend on it.
     const int tpt = W <= threads ? 1 : (W <= 2 * threads ? 2 : 4);
     const bool precise = P.flags & 1;
-#define FB_MS(G, TPT) k_multiple_scattering<G, false, TPT><<<grid, threads, lds, s>>>(P, T, J, dMS, S)
+#define FB_MS(G, TPT) k_multiple_scattering<G, false, TPT><<<grid, threads, lds, s>>>(P, MR, J, dMS, S)
     if (precise)
         tpt == 1 ? FB_MS(double, 1) : tpt == 2 ? FB_MS(double, 2) : FB_MS(double, 4);
     else

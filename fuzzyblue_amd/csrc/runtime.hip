@@ -101,6 +101,7 @@ Sizes sizes_of(const fb_parameters& p) {
     s.ni = size_t(p.irradiance_mu_s_size) * p.irradiance_r_size;
     s.ns = size_t(p.scattering_nu_size) * p.scattering_mu_s_size * p.scattering_mu_size *
            p.scattering_r_size;
+    s.nmr = size_t(p.scattering_mu_size) * p.scattering_r_size * fbk::kMsRowFloat4s;
     return s;
 }
 
@@ -163,6 +164,7 @@ void free_pending_scratch(fb_pending_atmosphere* pa) {
     free_buf(b, &pa->dM);
     free_buf(b, &pa->dMS);
     free_buf(b, &pa->J);
+    free_buf(b, &pa->MR);
 }
 
 void free_atmosphere_buffers(fb_atmosphere* a) {
@@ -193,6 +195,7 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
     if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dM);
     if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dMS);
     if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->J);
+    if (st == FB_OK) st = alloc_buf(b, s.nmr, &pa->MR);
     hipError_t e = hipSuccess;
     if (st == FB_OK) e = hipEventCreateWithFlags(&pa->done, hipEventDisableTiming);
     if (st == FB_OK && e == hipSuccess) e = hipEventCreateWithFlags(&a->ready, hipEventDisableTiming);
@@ -251,7 +254,7 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
     FB_MARK(FB_PASS_TRANSMITTANCE, 1);
     FB_HIP_TRY(fbk::launch_direct_irradiance(P, a->T.p, pa->dE.p, s));
     FB_MARK(FB_PASS_DIRECT_IRRADIANCE, 1);
-    FB_HIP_TRY(fbk::launch_single_scattering(P, a->T.p, pa->dR.p, pa->dM.p, a->S.p, s));
+    FB_HIP_TRY(fbk::launch_single_scattering(P, a->T.p, pa->dR.p, pa->dM.p, a->S.p, pa->MR.p, s));
     FB_MARK(FB_PASS_SINGLE_SCATTERING, 1);
     FB_HIP_TRY(fbk::launch_fill_zero(a->E.p, sz.ni, s));
     FB_MARK(FB_PASS_CLEAR_IRRADIANCE, 1);
@@ -262,7 +265,7 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
         FB_HIP_TRY(fbk::launch_indirect_irradiance(P, order - 1, pa->dR.p, pa->dM.p, pa->dMS.p,
                                                    pa->dE.p, a->E.p, s));
         FB_MARK(FB_PASS_INDIRECT_IRRADIANCE, order);
-        FB_HIP_TRY(fbk::launch_multiple_scattering(P, a->T.p, pa->J.p, pa->dMS.p, a->S.p, s));
+        FB_HIP_TRY(fbk::launch_multiple_scattering(P, pa->MR.p, pa->J.p, pa->dMS.p, a->S.p, s));
         FB_MARK(FB_PASS_MULTIPLE_SCATTERING, order);
     }
     return FB_OK;
@@ -444,7 +447,7 @@ size_t fb_parameters_footprint(const fb_parameters* p, size_t* resident, size_t*
     if (!p) return 0;
     Sizes s = sizes_of(*p);
     size_t res = (s.nt + s.ni + s.ns) * sizeof(float4);
-    size_t scr = (s.ni + 4 * s.ns) * sizeof(float4);
+    size_t scr = (s.ni + 4 * s.ns + s.nmr) * sizeof(float4);
     if (resident) *resident = res;
     if (scratch) *scratch = scr;
     return res + scr;

@@ -56,6 +56,7 @@ fbk::DevParams make_dev_params(const fb_parameters& p);
 
 struct Sizes {
     size_t nt, ni, ns;  // texel counts: transmittance, irradiance, scattering
+    size_t nmr;         // float4s of the multiple-scattering row table
 };
 Sizes sizes_of(const fb_parameters& p);
 
@@ -94,7 +95,7 @@ struct fb_atmosphere {
 struct fb_pending_atmosphere {
     fb_builder* builder = nullptr;
     fb_atmosphere* atm = nullptr;
-    fbrt::Buf dE, dR, dM, dMS, J;
+    fbrt::Buf dE, dR, dM, dMS, J, MR;
     hipEvent_t done = nullptr;
     hipGraphExec_t graph = nullptr;
     std::shared_ptr<void> batch;  // shared state of a batched build

@@ -114,6 +114,18 @@ class Parameters:
             self.scattering_r_size,
         )
 
+    def footprint(self):
+        """(resident, scratch) device bytes of one LUT set, as the library
+        allocates them (fb_parameters_footprint)."""
+        import ctypes as C
+
+        from .lib import load
+
+        res, scr = C.c_size_t(), C.c_size_t()
+        c = self.to_c()
+        load().fb_parameters_footprint(C.byref(c), C.byref(res), C.byref(scr))
+        return res.value, scr.value
+
     def replace(self, **kw) -> "Parameters":
         p = copy.deepcopy(self)
         for k, v in kw.items():

@@ -77,12 +77,7 @@ def test_budget_rejection_and_unknown_lut(builder):
 
 def test_lru_eviction(builder):
     p = Parameters().replace(**RESOLUTIONS["half"])
-    res = (p.transmittance_extent()[0] * p.transmittance_extent()[1]
-           + p.irradiance_extent()[0] * p.irradiance_extent()[1]
-           + p.scattering_extent()[0] * p.scattering_extent()[1] * p.scattering_extent()[2]) * 16
-    scratch = (p.irradiance_extent()[0] * p.irradiance_extent()[1]
-               + 4 * p.scattering_extent()[0] * p.scattering_extent()[1]
-               * p.scattering_extent()[2]) * 16
+    res, scratch = p.footprint()
     # room for two resident sets plus one build in flight
     s = Server(builder, 2 * res + res + scratch + 4096, max_inflight=1)
     luts = []
