@@ -41,6 +41,7 @@ from fuzzyblue_amd import (  # noqa: E402
     synchronize,
 )
 from fuzzyblue_amd.lib import check, load  # noqa: E402
+from fuzzyblue_amd.params import RESOLUTIONS  # noqa: E402
 
 METRIC = "LUT-sets/s + p50/p99 latency at ≤1e-3 rel err vs FP64, 1×MI355X"
 PROFILE_STEPS = 10
@@ -168,6 +169,10 @@ def measure_batch(b, stream, n=256, seed=4):
     from fuzzyblue_amd.workloads import mixed_batch
 
     ps = mixed_batch(n, seed, precise=True)
+    # One autotune per LUT-size class of the mix (block sizes only; the
+    # LUTs are bit-identical for every choice).
+    for res in RESOLUTIONS.values():
+        b.autotune(Parameters().replace(**res, flags=abi.FB_FLAG_PRECISE), stream)
     work = sum(costmodel.lut_set_work(p).roof_seconds() for p in ps)
     pend = Atmosphere.build_batch(b, stream, ps)
     for pa in pend:
@@ -317,6 +322,10 @@ def main():
     b = Builder(0)
     stream = Stream()
     p = Parameters().replace(flags=abi.FB_FLAG_PRECISE)
+    log("[bench] autotune")
+    tuning = b.autotune(p, stream)
+    tuning32 = b.autotune(p.replace(flags=0), stream)
+    log(f"[bench] tuning: {tuning}")
     log("[bench] precision-mode throughput")
     pa, ms = measure_throughput(b, stream, p, args.steps, args.warmup)
     value = args.steps / (ms * 1e-3)
@@ -400,6 +409,7 @@ def main():
             "precision": "precise",
         },
         "latency_ms": {"p50": p50, "p99": p99, "requests": args.latency_requests},
+        "tuning": {"precise": tuning, "fp32": tuning32},
         "max_rel_err_vs_fp64": err,
         "fp32_mode_lut_sets_s": args.steps / (ms32 * 1e-3),
         "sky": sky,

@@ -111,6 +111,23 @@ class fb_pass_timing(C.Structure):
     _fields_ = [("pass_", C.c_uint32), ("order", C.c_uint32), ("ms", C.c_float)]
 
 
+class fb_tuning(C.Structure):
+    _fields_ = [
+        ("single_scattering_threads", C.c_uint32),
+        ("density_order2_threads", C.c_uint32),
+        ("density_threads", C.c_uint32),
+        ("indirect_order1_waves", C.c_uint32),
+        ("indirect_waves", C.c_uint32),
+        ("multiple_scattering_threads", C.c_uint32),
+        ("best_ms", C.c_float * 6),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "best_ms"}
+        d["best_ms"] = list(self.best_ms)
+        return d
+
+
 class fb_lut_view(C.Structure):
     _fields_ = [
         ("device_ptr", C.c_void_p),

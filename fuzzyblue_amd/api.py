@@ -104,6 +104,34 @@ class Builder:
         self.handle = C.c_void_p()
         check(load().fb_builder_create(int(device), C.byref(self.handle)))
 
+    def autotune(self, params: Parameters, stream=None) -> dict:
+        """Time the candidate block sizes for params' LUT-size class and keep
+        the fastest (fb_builder_autotune)."""
+        t = abi.fb_tuning()
+        c = params.to_c()
+        check(load().fb_builder_autotune(self.handle, _stream_handle(stream), C.byref(c),
+                                         C.byref(t)))
+        return t.as_dict()
+
+    def tuning(self, params: Parameters):
+        """(tuned, block sizes) of params' size class (fb_builder_get_tuning)."""
+        t = abi.fb_tuning()
+        c = params.to_c()
+        st = load().fb_builder_get_tuning(self.handle, C.byref(c), C.byref(t))
+        if st not in (abi.FB_OK, abi.FB_ENOENT):
+            check(st)
+        return st == abi.FB_OK, t.as_dict()
+
+    def set_tuning(self, params: Parameters, **sizes):
+        """Install block sizes (fb_tuning field names; omitted = default)."""
+        t = abi.fb_tuning()
+        for k, v in sizes.items():
+            if k == "best_ms" or not hasattr(t, k):
+                raise AttributeError(k)
+            setattr(t, k, int(v))
+        c = params.to_c()
+        check(load().fb_builder_set_tuning(self.handle, C.byref(c), C.byref(t)))
+
     def close(self):
         if self.handle:
             load().fb_builder_destroy(self.handle)

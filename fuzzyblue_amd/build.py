@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libfuzzyblue_hip.so")
 
 ARCH = os.environ.get("FB_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["precompute.hip", "render.hip", "runtime.hip", "batch.hip", "server.hip"]
+SOURCES = ["precompute.hip", "render.hip", "runtime.hip", "batch.hip", "server.hip",
+           "tune.hip"]
 
 COMMON = [
     f"--offload-arch={ARCH}",

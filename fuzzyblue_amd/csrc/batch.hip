@@ -72,15 +72,33 @@ std::vector<int> tile_order(const std::vector<fb_parameters>& ps) {
 }
 
 fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
-                     const std::vector<fbk::DevParams>& dps, const std::vector<int>& order_idx,
-                     int pass, int order) {
+                     const std::vector<fbk::DevParams>& dps, const std::vector<fbk::Tuning>& tus,
+                     const std::vector<int>& order_idx, int pass, int order) {
     BatchLaunch l;
     l.pass = pass;
     l.order = order;
+    auto member = [&](int i) { return pass < fbk::kPassDensity || int(ps[i].order) >= order; };
+    // Block size of the tunable passes: the tuning of the launch's largest
+    // atmosphere when it suits every member, else a size that suits all.
+    unsigned threads = 0;
+    if (pass == fbk::kPassSingle || pass == fbk::kPassDensity || pass == fbk::kPassIndirect ||
+        pass == fbk::kPassMultiple) {
+        int first = -1;
+        for (int i : order_idx)
+            if (member(i)) {
+                first = i;
+                break;
+            }
+        if (first < 0) return FB_OK;
+        threads = fbk::tuned_threads(dps[first], pass, order, tus[first]);
+        for (int i : order_idx)
+            if (member(i) && !fbk::tuning_valid(dps[i], pass, order, threads))
+                threads = pass == fbk::kPassIndirect ? 64 : 256;
+    }
     std::vector<uint32_t> tiles;
     for (int i : order_idx) {
-        if (pass >= fbk::kPassDensity && int(ps[i].order) < order) continue;
-        fbk::PassShape sh = fbk::batch_pass_shape(dps[i], pass, order);
+        if (!member(i)) continue;
+        fbk::PassShape sh = fbk::batch_pass_shape(dps[i], pass, order, threads);
         if (sh.blocks >= (1u << 24)) {
             set_error("atmosphere %d needs %u blocks in pass %d (limit 2^24)", i, sh.blocks, pass);
             return FB_EINVAL;
@@ -140,12 +158,14 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
     const std::vector<int> order_idx = tile_order(ps);
     int max_order = 1;
     for (const auto& p : ps) max_order = std::max(max_order, int(p.order));
+    std::vector<fbk::Tuning> tus(n);
+    for (size_t i = 0; i < n; ++i) tus[i] = builder_tuning(b, ps[i]);
     for (int pass : {fbk::kPassTransmittance, fbk::kPassDirect, fbk::kPassSingle,
                      fbk::kPassClear})
-        if ((st = add_launch(*plan, ps, dps, order_idx, pass, 1)) != FB_OK) return fail(st);
+        if ((st = add_launch(*plan, ps, dps, tus, order_idx, pass, 1)) != FB_OK) return fail(st);
     for (int order = 2; order <= max_order; ++order)
         for (int pass : {fbk::kPassDensity, fbk::kPassIndirect, fbk::kPassMultiple})
-            if ((st = add_launch(*plan, ps, dps, order_idx, pass, order)) != FB_OK)
+            if ((st = add_launch(*plan, ps, dps, tus, order_idx, pass, order)) != FB_OK)
                 return fail(st);
 
     // Capture the schedule once; replays reuse it (fb_pending_relaunch).

@@ -47,14 +47,33 @@ enum BatchPass {
     kPassMultiple = 6
 };
 
+// Block sizes an autotuner may choose per LUT-size class; 0 = default.
+// None of them changes the arithmetic of a texel (each texel's sums run in
+// the same order whatever the block shape), so every choice gives
+// bit-identical LUTs.
+struct Tuning {
+    unsigned single = 0;     // threads per block, single scattering
+    unsigned density2 = 0;   // threads per block, density of order 2
+    unsigned density = 0;    // threads per block, density of order >= 3
+    unsigned indirect1 = 0;  // wavefronts per block, indirect irradiance of order 1
+    unsigned indirect = 0;   // wavefronts per block, indirect irradiance of order >= 2
+    unsigned multiple = 0;   // threads per block, multiple scattering
+};
+// `order` is the scattering order of the launch: density n, the indirect
+// pass of the order-n iteration (n - 1 bounces), multiple n.
+unsigned default_threads(const DevParams& P, int pass, int order);
+unsigned tuned_threads(const DevParams& P, int pass, int order, const Tuning& tu);
+bool tuning_valid(const DevParams& P, int pass, int order, unsigned threads);
+
 // Launch shape of one atmosphere's share of a batched pass: the number of
-// blocks (= tiles), threads per block and dynamic LDS bytes.
+// blocks (= tiles), threads per block and dynamic LDS bytes, for a block
+// size `threads` (used by the tunable passes).
 struct PassShape {
     unsigned blocks;
     unsigned threads;
     size_t lds;
 };
-PassShape batch_pass_shape(const DevParams& P, int pass, int order);
+PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned threads);
 
 // One launch of `pass` over a tile list (atmosphere slot << 24 | block).
 // `order` is the scattering order of the launch (density n, indirect n-1's
@@ -68,15 +87,18 @@ hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s);
 hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,
                                     hipStream_t s);
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
-                                    float4* dM, float4* S, float4* MR, hipStream_t s);
+                                    float4* dM, float4* S, float4* MR, const Tuning& tu,
+                                    hipStream_t s);
 hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
                                      const float4* dR, const float4* dM, const float4* dMS,
-                                     const float4* dE, float4* J, hipStream_t s);
+                                     const float4* dE, float4* J, const Tuning& tu,
+                                     hipStream_t s);
+// `order` here is the number of bounces summed (the loop's n - 1).
 hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float4* dR,
                                       const float4* dM, const float4* dMS, float4* dE, float4* E,
-                                      hipStream_t s);
+                                      const Tuning& tu, hipStream_t s);
 hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, const float4* J,
-                                      float4* dMS, float4* S, hipStream_t s);
+                                      float4* dMS, float4* S, const Tuning& tu, hipStream_t s);
 hipError_t launch_fill_zero(float4* p, size_t n, hipStream_t s);
 
 }  // namespace fbk

@@ -505,6 +505,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     // by every texel of the tile (texel x = j, y = 0).  Only the threads
     // that need them evaluate the G-precision maps, and the resulting taps
     // go through LDS, so no G value stays live across the gather loops.
+    const int slot0 = int(blockDim.x) >= 64 + nslots ? 64 : kDenTheta;
     if (tid < kDenTheta) {
         // Ground terms per theta row (scattering_density.comp:45-60).  The
         // ground normal's length does not depend on phi.
@@ -530,12 +531,13 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         s_ginv[l] = ginv;
         unsigned long long b = __ballot(g_l);
         if (l == 0) s_ground_mask = int(b & 0xffffull);
-    } else if (tid >= 64 && tid < 64 + nslots) {
-        // x taps of nu slot k = -1..NU (a second wavefront).
+    } else if (tid >= slot0 && tid < slot0 + nslots) {
+        // x taps of nu slot k = -1..NU (a second wavefront when the block
+        // has one).
         const TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
         const G u_mu_s = scattering_u_mu_s<G>(P, tc.mu_s);
-        s_tx[tid - 64] = slice_taps<G>(P, tid - 65, u_mu_s);
-        if (tid == 64) s_tz = linear_taps<G>(scattering_u_r<G>(P, tc.r), P.s_r);
+        s_tx[tid - slot0] = slice_taps<G>(P, tid - slot0 - 1, u_mu_s);
+        if (tid == slot0) s_tz = linear_taps<G>(scattering_u_r<G>(P, tc.r), P.s_r);
     }
     // delta_irradiance at r = bottom (irradiance.h:20-30 with x_r = 0).
     {
@@ -972,7 +974,7 @@ __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams*
     decode_tile(tiles[blockIdx.x], atm, local);
     const DevParams& P = Ps[atm];
     const LutPtrs& l = L[atm];
-    const int gx = (P.s_mu * P.s_nu + 255) / 256;
+    const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
     scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
                                         local / (gx * P.s_mus), l.T, l.dR, l.dM, l.dMS, l.dE, l.J);
 }
@@ -1052,10 +1054,9 @@ hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4*
 }
 
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
-                                    float4* dM, float4* S, float4* MR, hipStream_t s) {
-    const int W = P.s_nu * P.s_mus;
-    int threads = ((W + 63) / 64) * 64;
-    threads = threads > 256 ? 256 : threads;
+                                    float4* dM, float4* S, float4* MR, const Tuning& tu,
+                                    hipStream_t s) {
+    const unsigned threads = tuned_threads(P, kPassSingle, 2, tu);
     dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
         k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR);
@@ -1066,32 +1067,33 @@ hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4*
 
 hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
                                      const float4* dR, const float4* dM, const float4* dMS,
-                                     const float4* dE, float4* J, hipStream_t s) {
-    dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, 256), P.s_mus, P.s_r);
+                                     const float4* dE, float4* J, const Tuning& tu,
+                                     hipStream_t s) {
+    const unsigned threads = tuned_threads(P, kPassDensity, order, tu);
+    dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, threads), P.s_mus, P.s_r);
     // The density of order n reads the scattering of order n-1; order 1 is
     // single Rayleigh + Mie (scattering.h:157-179).
     const bool single = (order - 1) == 1;
     size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
     const bool precise = P.flags & 1;
     if (precise && single)
-        k_scattering_density<double, true><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
     else if (precise)
-        k_scattering_density<double, false><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
     else if (single)
-        k_scattering_density<float, true><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
     else
-        k_scattering_density<float, false><<<grid, 256, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
     return hipGetLastError();
 }
 
 hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float4* dR,
                                       const float4* dM, const float4* dMS, float4* dE, float4* E,
-                                      hipStream_t s) {
+                                      const Tuning& tu, hipStream_t s) {
     size_t n = size_t(P.i_mus) * P.i_r;
     const bool single = order == 1;
     size_t per_wave = indirect_lds_bytes_per_wave(P.s_nu, single);
-    int waves = int(65536 / per_wave);
-    waves = waves < 1 ? 1 : (waves > kIndWaves ? kIndWaves : waves);
+    const int waves = int(tuned_threads(P, kPassIndirect, order + 1, tu) / 64);
     size_t lds = per_wave * waves;
     dim3 grid(cdiv(n, waves)), block(64 * waves);
     const bool precise = P.flags & 1;
@@ -1107,10 +1109,9 @@ hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float
 }
 
 hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, const float4* J,
-                                      float4* dMS, float4* S, hipStream_t s) {
+                                      float4* dMS, float4* S, const Tuning& tu, hipStream_t s) {
     const int W = P.s_nu * P.s_mus;
-    int threads = ((W + 63) / 64) * 64;
-    threads = threads > 256 ? 256 : threads;
+    const int threads = int(tuned_threads(P, kPassMultiple, 2, tu));
     if (W > threads * kMsTexelsPerThread) return hipErrorInvalidValue;
     dim3 grid(P.s_mu, P.s_r);
     size_t lds = sizeof(float4) * 2 * size_t(W);
@@ -1127,8 +1128,58 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, cons
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ tuning
+// Default and tuned block sizes.  `order` is the launch's scattering order
+// (density n, the indirect pass of the order-n loop iteration, multiple n).
+unsigned default_threads(const DevParams& P, int pass, int order) {
+    const unsigned W = unsigned(P.s_nu * P.s_mus);
+    const unsigned row = std::min(256u, (W + 63) / 64 * 64);
+    switch (pass) {
+        case kPassSingle:
+        case kPassMultiple:
+            return row;
+        case kPassDensity:
+            return 256;
+        case kPassIndirect: {
+            const size_t per_wave = indirect_lds_bytes_per_wave(P.s_nu, (order - 1) == 1);
+            size_t waves = 65536 / per_wave;
+            waves = waves < 1 ? 1 : (waves > size_t(kIndWaves) ? size_t(kIndWaves) : waves);
+            return unsigned(64 * waves);
+        }
+    }
+    return 256;
+}
+
+unsigned tuned_threads(const DevParams& P, int pass, int order, const Tuning& tu) {
+    unsigned t = 0;
+    switch (pass) {
+        case kPassSingle: t = tu.single; break;
+        case kPassDensity: t = order == 2 ? tu.density2 : tu.density; break;
+        case kPassIndirect: t = 64 * (order == 2 ? tu.indirect1 : tu.indirect); break;
+        case kPassMultiple: t = tu.multiple; break;
+    }
+    return (t != 0 && tuning_valid(P, pass, order, t)) ? t : default_threads(P, pass, order);
+}
+
+bool tuning_valid(const DevParams& P, int pass, int order, unsigned threads) {
+    if (threads == 0 || threads % 64 != 0 || threads > 256) return false;
+    const unsigned W = unsigned(P.s_nu * P.s_mus);
+    switch (pass) {
+        case kPassSingle:
+            return true;
+        case kPassMultiple:
+            return W <= threads * unsigned(kMsTexelsPerThread);
+        case kPassDensity:
+            return true;
+        case kPassIndirect:
+            return indirect_lds_bytes_per_wave(P.s_nu, (order - 1) == 1) * (threads / 64) <=
+                   65536;
+    }
+    return false;
+}
+
 // ------------------------------------------------------------ batched
-PassShape batch_pass_shape(const DevParams& P, int pass, int order) {
+PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned threads) {
     const size_t ni = size_t(P.i_mus) * P.i_r;
     const bool precise = P.flags & 1;
     switch (pass) {
@@ -1139,16 +1190,19 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order) {
         case kPassDirect:
             return {cdiv(P.i_mus, 64) * unsigned(P.i_r), 64, 0};
         case kPassSingle:
-            return {unsigned(P.s_mu) * unsigned(P.s_r), 256, 0};
+            return {unsigned(P.s_mu) * unsigned(P.s_r), threads, 0};
         case kPassClear:
             return {cdiv(ni, 256), 256, 0};
         case kPassDensity:
-            return {cdiv(size_t(P.s_mu) * P.s_nu, 256) * unsigned(P.s_mus) * unsigned(P.s_r), 256,
-                    density_lds_bytes(P.s_nu, P.i_mus, (order - 1) == 1)};
-        case kPassIndirect:  // one wave per texel, one wave per block
-            return {unsigned(ni), 64, indirect_lds_bytes_per_wave(P.s_nu, (order - 1) == 1)};
+            return {cdiv(size_t(P.s_mu) * P.s_nu, threads) * unsigned(P.s_mus) * unsigned(P.s_r),
+                    threads, density_lds_bytes(P.s_nu, P.i_mus, (order - 1) == 1)};
+        case kPassIndirect: {  // one wave per texel
+            const unsigned waves = threads / 64;
+            return {cdiv(ni, waves), threads,
+                    indirect_lds_bytes_per_wave(P.s_nu, (order - 1) == 1) * waves};
+        }
         case kPassMultiple:
-            return {unsigned(P.s_mu) * unsigned(P.s_r), 256,
+            return {unsigned(P.s_mu) * unsigned(P.s_r), threads,
                     sizeof(float4) * 2 * size_t(P.s_nu) * P.s_mus};
     }
     return {0, 0, 0};

@@ -249,23 +249,26 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
                              PassTrace* trace) {
     const fbk::DevParams& P = a->dev;
     Sizes sz = sizes_of(a->params);
+    const fbk::Tuning tu = builder_tuning(a->builder, a->params);
     FB_MARK(0, 0);
     FB_HIP_TRY(fbk::launch_transmittance(P, a->T.p, s));
     FB_MARK(FB_PASS_TRANSMITTANCE, 1);
     FB_HIP_TRY(fbk::launch_direct_irradiance(P, a->T.p, pa->dE.p, s));
     FB_MARK(FB_PASS_DIRECT_IRRADIANCE, 1);
-    FB_HIP_TRY(fbk::launch_single_scattering(P, a->T.p, pa->dR.p, pa->dM.p, a->S.p, pa->MR.p, s));
+    FB_HIP_TRY(
+        fbk::launch_single_scattering(P, a->T.p, pa->dR.p, pa->dM.p, a->S.p, pa->MR.p, tu, s));
     FB_MARK(FB_PASS_SINGLE_SCATTERING, 1);
     FB_HIP_TRY(fbk::launch_fill_zero(a->E.p, sz.ni, s));
     FB_MARK(FB_PASS_CLEAR_IRRADIANCE, 1);
     for (int order = 2; order <= int(a->params.order); ++order) {
         FB_HIP_TRY(fbk::launch_scattering_density(P, order, a->T.p, pa->dR.p, pa->dM.p,
-                                                  pa->dMS.p, pa->dE.p, pa->J.p, s));
+                                                  pa->dMS.p, pa->dE.p, pa->J.p, tu, s));
         FB_MARK(FB_PASS_SCATTERING_DENSITY, order);
         FB_HIP_TRY(fbk::launch_indirect_irradiance(P, order - 1, pa->dR.p, pa->dM.p, pa->dMS.p,
-                                                   pa->dE.p, a->E.p, s));
+                                                   pa->dE.p, a->E.p, tu, s));
         FB_MARK(FB_PASS_INDIRECT_IRRADIANCE, order);
-        FB_HIP_TRY(fbk::launch_multiple_scattering(P, pa->MR.p, pa->J.p, pa->dMS.p, a->S.p, s));
+        FB_HIP_TRY(
+            fbk::launch_multiple_scattering(P, pa->MR.p, pa->J.p, pa->dMS.p, a->S.p, tu, s));
         FB_MARK(FB_PASS_MULTIPLE_SCATTERING, order);
     }
     return FB_OK;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <atomic>
 #include <map>
 #include <memory>
@@ -68,6 +69,12 @@ Sizes sizes_of(const fb_parameters& p);
 // (src/precompute.rs:1045-1047, 1077): every atmosphere and pending
 // atmosphere holds a reference, fb_builder_destroy drops the caller's, and
 // the device state is released with the last one.
+namespace fbrt {
+// Tuning-table key: the precision flag and the eight LUT sizes.
+using TuneKey = std::array<uint32_t, 9>;
+TuneKey tune_key(const fb_parameters& p);
+}  // namespace fbrt
+
 struct fb_builder {
     int device = 0;
     hipDeviceProp_t prop{};
@@ -75,11 +82,15 @@ struct fb_builder {
     hipStream_t capture_stream = nullptr;  // private stream for graph capture
     std::mutex mu;
     std::atomic<int> refs{1};
+    std::mutex tune_mu;
+    std::map<fbrt::TuneKey, fbk::Tuning> tuning;  // per LUT-size class (tune.hip)
 };
 
 namespace fbrt {
 void builder_retain(fb_builder* b);
 void builder_release(fb_builder* b);
+// The builder's block sizes for a parameter set (defaults when untuned).
+fbk::Tuning builder_tuning(fb_builder* b, const fb_parameters& p);
 }  // namespace fbrt
 
 struct fb_atmosphere {

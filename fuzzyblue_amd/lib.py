@@ -58,6 +58,9 @@ def load():
         "fb_atmosphere_build": (S, [V, V, P, C.POINTER(V)]),
         "fb_atmosphere_build_batch": (S, [V, V, P, C.c_size_t, V]),
         "fb_pending_relaunch": (S, [V, V]),
+        "fb_builder_autotune": (S, [V, V, P, C.POINTER(abi.fb_tuning)]),
+        "fb_builder_get_tuning": (S, [V, P, C.POINTER(abi.fb_tuning)]),
+        "fb_builder_set_tuning": (S, [V, P, C.POINTER(abi.fb_tuning)]),
         "fb_pending_profile": (
             S, [V, V, C.POINTER(abi.fb_pass_timing), C.c_size_t, C.POINTER(C.c_size_t)]),
         "fb_pending_peek": (V, [V]),

@@ -165,6 +165,36 @@ fb_status fb_atmosphere_build_batch(fb_builder *b, fb_stream stream,
  * benches/precompute.rs:138-148).  Uses the captured hipGraph. */
 fb_status fb_pending_relaunch(fb_pending_atmosphere *pa, fb_stream stream);
 
+/* Block sizes of the tunable passes for one LUT-size class (the precision
+ * flag and the eight sizes of fb_parameters).  No reference counterpart:
+ * the reference's shaders fix their workgroup sizes (8x8, 4x4x4).  Every
+ * choice gives bit-identical LUTs; builds (plain and batched) started
+ * after the tuning is set use it. */
+typedef struct fb_tuning {
+    uint32_t single_scattering_threads;   /* 64, 128 or 256 per (r, mu) row */
+    uint32_t density_order2_threads;      /* 64, 128 or 256 per (r, mu_s) tile */
+    uint32_t density_threads;             /* the same for orders >= 3 */
+    uint32_t indirect_order1_waves;       /* 1, 2 or 4 wavefronts per block */
+    uint32_t indirect_waves;              /* the same for orders >= 2 */
+    uint32_t multiple_scattering_threads; /* 64, 128 or 256 per (r, mu) row */
+    float best_ms[6]; /* fb_builder_autotune: measured ms per launch of each
+                         chosen shape, in field order; 0 otherwise */
+} fb_tuning;
+
+/* Time every candidate block size of each tunable pass on a scratch LUT
+ * set of p's sizes (enqueued on `stream`, synchronised), keep the fastest
+ * in the builder's table, and return the choice in *out (may be NULL). */
+fb_status fb_builder_autotune(fb_builder *b, fb_stream stream,
+                              const fb_parameters *p, fb_tuning *out);
+/* The block sizes builds of p's size class use; FB_ENOENT (with the
+ * defaults in *out) when the class was never tuned. */
+fb_status fb_builder_get_tuning(fb_builder *b, const fb_parameters *p,
+                                fb_tuning *out);
+/* Install block sizes (0 = default) for p's size class, e.g. a table
+ * saved from an earlier autotune; FB_EINVAL if one is not valid there. */
+fb_status fb_builder_set_tuning(fb_builder *b, const fb_parameters *p,
+                                const fb_tuning *t);
+
 /* Pass identifiers of the precompute schedule (src/precompute.rs:1726-1999). */
 typedef enum fb_pass {
     FB_PASS_TRANSMITTANCE = 0,       /* shaders/transmittance.comp */
