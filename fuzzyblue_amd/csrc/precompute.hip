@@ -396,13 +396,12 @@ struct DenTexel {
     float osx, osy, osz;     // omega_s
     float g1, g2;            // 1 + g^2, -2g
     float hnu, hE;           // (NU-1)/2, (IRR_MUS-1)/2
-    int kmax, emax;          // NU-1, IRR_MUS-1
+    float kmax, emax;        // NU-1, IRR_MUS-1
     const ProfEntry* erow;   // delta_irradiance at r = bottom
 };
 // Per-(texel, theta) constants.
 struct DenRow {
     float C, omzc;           // cos(theta)*omega_s.z, cos(theta)*omega.z
-    float3 rr, mm;           // kR*beta_R*rho_R*domega, kM*beta_M*rho_M*domega
     const ProfEntry* pr;     // nu profile (order n-1 >= 2: multiple; 1: rayleigh)
     const ProfEntry* pm;     // nu profile of single Mie (order 1)
     const float4* dir;       // c_den_dir row: (cos phi sin t, sin phi sin t, cos t, dw)
@@ -410,12 +409,17 @@ struct DenRow {
     float Ag, Bg, Cg;        // ground normal . omega_s = cx*Ag + sy*Bg + Cg
 };
 
-// Entry address of texel coordinate t: floor clamped to [-1, kmax].
+// Linear filtering at texel coordinate t, clamped to [0, kmax] first.  For
+// t in range this is exactly the CLAMP_TO_EDGE result of prof_lerp (at
+// t = kmax the delta has weight 0); a coordinate that rounding put just
+// outside the range (|nu| a few ulp above 1) reads the end value.  With
+// t >= 0, truncation is floor, so the slot and weight take one conversion
+// and one fract.
 __device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof, float t,
-                                              int kmax) {
-    int k = int(floorf(t));
-    float f = __builtin_amdgcn_fractf(t);
-    k = min(max(k, -1), kmax);
+                                              float kmax) {
+    t = __builtin_amdgcn_fmed3f(t, 0.0f, kmax);
+    const int k = int(t);
+    const float f = __builtin_amdgcn_fractf(t);
     const ProfEntry* e = prof + (k + 1);
     float4 a = e->a;
     float2 b = e->b;
@@ -424,18 +428,20 @@ __device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof
 
 // The 16 direction pairs (phi_m, phi_{31-m}) of one theta row.  Both members
 // share cos(phi), hence nu2 = omega.omega_i and the phase functions; their
-// sin(phi) terms have opposite signs.
+// sin(phi) terms have opposite signs.  The row's Rayleigh and Mie sums are
+// kept apart (aR = sum of q L, aM = sum of q w L, with q = 1 + nu2^2 and w
+// the Cornette-Shanks power): their scattering coefficients are constant
+// over the row and are applied once by the caller.
 template <bool SINGLE, bool GROUND>
-__device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& row, float3& acc) {
+__device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& row, float3& aR,
+                                            float3& aM) {
 #pragma unroll
     for (int m = 0; m < kDenPhi / 2; ++m) {
         const float4 o = row.dir[m];
         const float cx = o.x, sy = o.y;
         const float nu2 = fmaf(cx, dt.omx, row.omzc);
         const float q = fmaf(nu2, nu2, 1.0f);
-        const float w = rsqrt_cubed(fmaf(dt.g2, nu2, dt.g1));
-        const float u0 = q * fmaf(row.mm.x, w, row.rr.x), u1 = q * fmaf(row.mm.y, w, row.rr.y),
-                    u2 = q * fmaf(row.mm.z, w, row.rr.z);
+        const float qw = q * rsqrt_cubed(fmaf(dt.g2, nu2, dt.g1));
         const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
         const float nua = base + sb, nub = base - sb;
         const float ta = fmaf(nua, dt.hnu, dt.hnu), tb = fmaf(nub, dt.hnu, dt.hnu);
@@ -462,9 +468,13 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
             Lb = make_float3(fmaf(row.gv.x, Eb.x, Lb.x), fmaf(row.gv.y, Eb.y, Lb.y),
                              fmaf(row.gv.z, Eb.z, Lb.z));
         }
-        acc.x = fmaf(La.x + Lb.x, u0, acc.x);
-        acc.y = fmaf(La.y + Lb.y, u1, acc.y);
-        acc.z = fmaf(La.z + Lb.z, u2, acc.z);
+        const float Sx = La.x + Lb.x, Sy = La.y + Lb.y, Sz = La.z + Lb.z;
+        aR.x = fmaf(q, Sx, aR.x);
+        aR.y = fmaf(q, Sy, aR.y);
+        aR.z = fmaf(q, Sz, aR.z);
+        aM.x = fmaf(qw, Sx, aM.x);
+        aM.y = fmaf(qw, Sy, aM.y);
+        aM.z = fmaf(qw, Sz, aM.z);
     }
 }
 
@@ -612,19 +622,18 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     dt.g2 = -2.0f * g;
     dt.hnu = 0.5f * float(NU - 1);
     dt.hE = 0.5f * float(NE - 1);
-    dt.kmax = NU - 1;
-    dt.emax = NE - 1;
+    dt.kmax = float(NU - 1);
+    dt.emax = float(NE - 1);
     dt.erow = s_erow;
 
     float3 acc = make_float3(0.0f, 0.0f, 0.0f);
     for (int l = 0; l < kDenTheta; ++l) {
+        float3 aR = make_float3(0.0f, 0.0f, 0.0f), aM = make_float3(0.0f, 0.0f, 0.0f);
         const float4 d0 = c_den_dir[l * kDenPhi];
         const float ct = d0.z, dom = d0.w;
         DenRow row;
         row.C = ct * dt.osz;
         row.omzc = ct * dt.omz;
-        row.rr = make_float3(cr0 * dom, cr1 * dom, cr2 * dom);
-        row.mm = make_float3(cm0 * dom, cm1 * dom, cm2 * dom);
         row.pr = s_prof + l * nent;
         row.pm = s_prof + (kDenTheta + l) * nent;
         row.dir = c_den_dir + l * kDenPhi;
@@ -636,10 +645,14 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
             row.Ag = dgi * dt.osx;
             row.Bg = dgi * dt.osy;
             row.Cg = (rf + ct * gv.w) * inv * dt.osz;
-            density_row<SINGLE, true>(dt, row, acc);
+            density_row<SINGLE, true>(dt, row, aR, aM);
         } else {
-            density_row<SINGLE, false>(dt, row, acc);
+            density_row<SINGLE, false>(dt, row, aR, aM);
         }
+        // Scattering coefficients x phase constants x solid angle.
+        acc.x = fmaf(cm0 * dom, aM.x, fmaf(cr0 * dom, aR.x, acc.x));
+        acc.y = fmaf(cm1 * dom, aM.y, fmaf(cr1 * dom, aR.y, acc.y));
+        acc.z = fmaf(cm2 * dom, aM.z, fmaf(cr2 * dom, aR.z, acc.z));
     }
     J[(size_t(z) * MU + y) * W + x] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }
