@@ -96,6 +96,14 @@ class fb_parameters(C.Structure):
     ]
 
 
+FB_PASS_TRANSMITTANCE = 0
+FB_PASS_DIRECT_IRRADIANCE = 1
+FB_PASS_SINGLE_SCATTERING = 2
+FB_PASS_CLEAR_IRRADIANCE = 3
+FB_PASS_SCATTERING_DENSITY = 4
+FB_PASS_INDIRECT_IRRADIANCE = 5
+FB_PASS_MULTIPLE_SCATTERING = 6
+
 FB_PASS_NAMES = {
     0: "transmittance",
     1: "direct_irradiance",
@@ -119,6 +127,7 @@ class fb_tuning(C.Structure):
         ("indirect_order1_waves", C.c_uint32),
         ("indirect_waves", C.c_uint32),
         ("multiple_scattering_threads", C.c_uint32),
+        ("xcd_remap", C.c_uint32),
         ("best_ms", C.c_float * 6),
     ]
 

@@ -58,6 +58,7 @@ struct Tuning {
     unsigned indirect1 = 0;  // wavefronts per block, indirect irradiance of order 1
     unsigned indirect = 0;   // wavefronts per block, indirect irradiance of order >= 2
     unsigned multiple = 0;   // threads per block, multiple scattering
+    unsigned remap = 0;      // bit p (a BatchPass): XCD-contiguous block order
 };
 // `order` is the scattering order of the launch: density n, the indirect
 // pass of the order-n iteration (n - 1 bounces), multiple n.

@@ -762,13 +762,19 @@ __device__ __forceinline__ void indirect_irradiance_block(const DevParams& P, co
 // each texel then lerps 2 nu slices x 2 mu_s taps from LDS.
 constexpr int kMsTexelsPerThread = 4;
 
+// A staged density value: rgb only (12 bytes, 4-byte aligned), so the
+// four taps of a sample cost 48 bytes of LDS reads, not 64.
+struct Rgb {
+    float x, y, z;
+};
+
 template <typename G, bool COMP, int TPT>
 __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                              const float4* __restrict__ MR,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
                                                              float4* __restrict__ S) {
-    extern __shared__ float4 s_row[];           // [2][W]
+    extern __shared__ Rgb s_row[];              // [2][W + MUS + 1]: staged density rows
     __shared__ float4 s_desc[kMsRowFloat4s];    // the row's entries of the table MR
     const float4* s_tw = s_desc;                // T(r,mu,d_i)*dx*weight (rgb), d_i
     const float4* s_geo = s_desc + kMsSamples;  // 1/r_i, wy, wz
@@ -825,7 +831,16 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         const int4 o = s_off[i];
         return {ldg4(J + o.x + x), ldg4(J + o.y + x), ldg4(J + o.z + x), ldg4(J + o.w + x)};
     };
-    auto put = [&](int i, const Rows& q, float4* dst, int x) {
+    // Row layout: entries 0..W-1, then entry W = entry W-1 (the right edge
+    // of CLAMP_TO_EDGE), then MUS zero entries that only ever carry weight
+    // 0 (the nu slice above the last one, where the nu lerp weight is 0).
+    // The padding lets every tap index go unclamped.
+    const int WP = W + MUS + 1;
+    for (int e = tid; e < MUS; e += nt) {
+        s_row[W + 1 + e] = Rgb{0.0f, 0.0f, 0.0f};
+        s_row[WP + W + 1 + e] = Rgb{0.0f, 0.0f, 0.0f};
+    }
+    auto put = [&](int i, const Rows& q, Rgb* dst, int x) {
         const float4 gq = s_geo[i];
         float ax = fmaf(gq.y, q.r01.x - q.r00.x, q.r00.x),
               ay = fmaf(gq.y, q.r01.y - q.r00.y, q.r00.y),
@@ -833,11 +848,12 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         float bx = fmaf(gq.y, q.r11.x - q.r10.x, q.r10.x),
               by = fmaf(gq.y, q.r11.y - q.r10.y, q.r10.y),
               bz = fmaf(gq.y, q.r11.z - q.r10.z, q.r10.z);
-        dst[x] = make_float4(fmaf(gq.z, bx - ax, ax), fmaf(gq.z, by - ay, ay),
-                             fmaf(gq.z, bz - az, az), 0.0f);
+        const Rgb v{fmaf(gq.z, bx - ax, ax), fmaf(gq.z, by - ay, ay), fmaf(gq.z, bz - az, az)};
+        dst[x] = v;
+        if (x == W - 1) dst[W] = v;
     };
     // Sample i of every texel of this thread, from the staged row `cur`.
-    auto eval = [&](int i, const float4* __restrict__ cur) {
+    auto eval = [&](int i, const Rgb* __restrict__ cur) {
         const float4 tw = s_tw[i];
         const float inv_r_i = s_geo[i].x, d_i = tw.w;
 #pragma unroll
@@ -850,15 +866,18 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
                 float disc = fmaf(b2, mu_s_i * mu_s_i, H2);
                 float d = fmaxf(fmaf(-b, mu_s_i, __builtin_amdgcn_sqrtf(disc)), 0.0f);
                 float a = (d - d_min) * inv_range;
-                float xm = fmaxf(fmaf(-a, invA, 1.0f), 0.0f) * __builtin_amdgcn_rcpf(1.0f + a);
+                // max(1 - a/A, 0) / (1 + a), kept <= 1 (a >= 0 up to rounding).
+                float xm = __builtin_amdgcn_fmed3f(
+                    fmaf(-a, invA, 1.0f) * __builtin_amdgcn_rcpf(1.0f + a), 0.0f, 1.0f);
                 float xt = fmaf(xm, musm1, tslice[q] * fMUS);
                 float fl = floorf(xt);
                 float w = xt - fl;
                 int i0 = int(fl);
-                // xt >= 0 (xm >= 0, tslice >= 0): only the right edge clamps.
-                int c0 = min(i0, W - 1), c1 = min(i0 + 1, W - 1);
-                int c2 = min(i0 + MUS, W - 1), c3 = min(i0 + MUS + 1, W - 1);
-                float4 p0 = cur[c0], p1 = cur[c1], p2 = cur[c2], p3 = cur[c3];
+                // 0 <= i0 <= W - 1 (0 <= xm <= 1 up to rounding, 0 <= tslice
+                // <= NU - 1): the taps i0, i0+1, i0+MUS, i0+MUS+1 stay inside
+                // the padded row (see the layout above).
+                const Rgb p0 = cur[i0], p1 = cur[i0 + 1], p2 = cur[i0 + MUS],
+                          p3 = cur[i0 + MUS + 1];
                 float l = tlerp[q];
                 float v0 = fmaf(w, p1.x - p0.x, p0.x), v1 = fmaf(w, p1.y - p0.y, p0.y),
                       v2 = fmaf(w, p1.z - p0.z, p0.z);
@@ -873,8 +892,8 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
     for (int x = tid; x < W; x += nt) put(0, fetch(0, x), s_row, x);
     __syncthreads();
     for (int i = 0; i < kMsSamples; ++i) {
-        const float4* cur = s_row + (i & 1) * W;
-        float4* nxt = s_row + ((i + 1) & 1) * W;
+        const Rgb* cur = s_row + (i & 1) * WP;
+        Rgb* nxt = s_row + ((i + 1) & 1) * WP;
         const bool next = i + 1 < kMsSamples;
         if constexpr (TPT <= 2) {
             // Issue the loads of sample i+1 first, so that their latency
@@ -951,13 +970,30 @@ __global__ __launch_bounds__(256) void kb_direct_irradiance(const DevParams* __r
     direct_irradiance_block<G>(P, local % gx, local / gx, 0, L[atm].T, L[atm].dE);
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has
+// its own L2.  With `remap` set, XCD k instead runs the k-th contiguous
+// share of the logical blocks, so blocks of neighbouring rows / tiles, which
+// read overlapping LUT rows, share an L2.  A bijection on [0, N).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_remap(unsigned L, unsigned N) {
+    const unsigned xcd = L % kXcds, local = L / kXcds;
+    const unsigned per = N / kXcds, rem = N % kXcds;
+    return xcd * per + min(xcd, rem) + local;
+}
+__device__ __forceinline__ unsigned logical_block(bool remap) {
+    const unsigned L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    return remap ? xcd_remap(L, gridDim.x * gridDim.y * gridDim.z) : L;
+}
+
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const float4* __restrict__ T,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
                                                            float4* __restrict__ S,
-                                                           float4* __restrict__ MR) {
-    single_scattering_block<G, COMP>(P, blockIdx.x, blockIdx.y, 0, T, dR, dM, S, MR);
+                                                           float4* __restrict__ MR,
+                                                           unsigned remap) {
+    const unsigned b = logical_block(remap);
+    single_scattering_block<G, COMP>(P, b % P.s_mu, b / P.s_mu, 0, T, dR, dM, S, MR);
 }
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
@@ -975,9 +1011,10 @@ template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256, 8) void k_scattering_density(
     DevParams P, const float4* __restrict__ T, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
-    float4* __restrict__ J) {
-    scattering_density_block<G, SINGLE>(P, blockIdx.x, blockIdx.y, blockIdx.z, T, dR, dM, dMS, dE,
-                                        J);
+    float4* __restrict__ J, unsigned remap) {
+    const unsigned b = logical_block(remap), gx = gridDim.x;
+    scattering_density_block<G, SINGLE>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus), T, dR,
+                                        dM, dMS, dE, J);
 }
 template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams* __restrict__ Ps,
@@ -1013,8 +1050,10 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
                                                              const float4* __restrict__ MR,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
-                                                             float4* __restrict__ S) {
-    multiple_scattering_block<G, COMP, TPT>(P, blockIdx.x, blockIdx.y, 0, MR, J, dMS, S);
+                                                             float4* __restrict__ S,
+                                                             unsigned remap) {
+    const unsigned b = logical_block(remap);
+    multiple_scattering_block<G, COMP, TPT>(P, b % P.s_mu, b / P.s_mu, 0, MR, J, dMS, S);
 }
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
@@ -1070,11 +1109,12 @@ hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4*
                                     float4* dM, float4* S, float4* MR, const Tuning& tu,
                                     hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassSingle, 2, tu);
+    const unsigned remap = (tu.remap >> kPassSingle) & 1u;
     dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
-        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR);
+        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR, remap);
     else
-        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR);
+        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR, remap);
     return hipGetLastError();
 }
 
@@ -1083,6 +1123,7 @@ hipError_t launch_scattering_density(const DevParams& P, int order, const float4
                                      const float4* dE, float4* J, const Tuning& tu,
                                      hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassDensity, order, tu);
+    const unsigned remap = (tu.remap >> kPassDensity) & 1u;
     dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, threads), P.s_mus, P.s_r);
     // The density of order n reads the scattering of order n-1; order 1 is
     // single Rayleigh + Mie (scattering.h:157-179).
@@ -1090,13 +1131,13 @@ hipError_t launch_scattering_density(const DevParams& P, int order, const float4
     size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
     const bool precise = P.flags & 1;
     if (precise && single)
-        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
     else if (precise)
-        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
     else if (single)
-        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
     else
-        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J);
+        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
     return hipGetLastError();
 }
 
@@ -1125,14 +1166,16 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, cons
                                       float4* dMS, float4* S, const Tuning& tu, hipStream_t s) {
     const int W = P.s_nu * P.s_mus;
     const int threads = int(tuned_threads(P, kPassMultiple, 2, tu));
+    const unsigned remap = (tu.remap >> kPassMultiple) & 1u;
     if (W > threads * kMsTexelsPerThread) return hipErrorInvalidValue;
     dim3 grid(P.s_mu, P.s_r);
-    size_t lds = sizeof(float4) * 2 * size_t(W);
+    size_t lds = sizeof(Rgb) * 2 * size_t(W + P.s_mus + 1);
     // Texels per thread: 1 up to 256 texels per row, else 2 or 4.  The
     // arithmetic of a texel does not depend on it.
     const int tpt = W <= threads ? 1 : (W <= 2 * threads ? 2 : 4);
     const bool precise = P.flags & 1;
-#define FB_MS(G, TPT) k_multiple_scattering<G, false, TPT><<<grid, threads, lds, s>>>(P, MR, J, dMS, S)
+#define FB_MS(G, TPT) \
+    k_multiple_scattering<G, false, TPT><<<grid, threads, lds, s>>>(P, MR, J, dMS, S, remap)
     if (precise)
         tpt == 1 ? FB_MS(double, 1) : tpt == 2 ? FB_MS(double, 2) : FB_MS(double, 4);
     else
@@ -1216,7 +1259,7 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
         }
         case kPassMultiple:
             return {unsigned(P.s_mu) * unsigned(P.s_r), threads,
-                    sizeof(float4) * 2 * size_t(P.s_nu) * P.s_mus};
+                    sizeof(Rgb) * 2 * (size_t(P.s_nu) * P.s_mus + P.s_mus + 1)};
     }
     return {0, 0, 0};
 }

@@ -43,6 +43,7 @@ fb_tuning to_public(const fbk::DevParams& P, const fbk::Tuning& t) {
     o.indirect_order1_waves = fbk::tuned_threads(P, fbk::kPassIndirect, 2, t) / 64;
     o.indirect_waves = fbk::tuned_threads(P, fbk::kPassIndirect, 3, t) / 64;
     o.multiple_scattering_threads = fbk::tuned_threads(P, fbk::kPassMultiple, 2, t);
+    o.xcd_remap = t.remap;
     return o;
 }
 
@@ -96,33 +97,51 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
     fbk::Tuning best = builder_tuning(b, *p);
     float best_ms[6] = {0, 0, 0, 0, 0, 0};
 
-    // Time `launch(tuning)` for each valid candidate of one knob and keep
-    // the fastest in best.*field.
+    // Mean ms per launch of `launch(t)` after warm-up.
+    auto time = [&](const fbk::Tuning& t, auto launch, float* ms) -> fb_status {
+        for (int k = 0; k < kWarm; ++k) FB_HIP_TRY(launch(t));
+        FB_HIP_TRY(hipEventRecord(ev.a, s));
+        for (int k = 0; k < kReps; ++k) FB_HIP_TRY(launch(t));
+        FB_HIP_TRY(hipEventRecord(ev.b, s));
+        FB_HIP_TRY(hipEventSynchronize(ev.b));
+        FB_HIP_TRY(hipEventElapsedTime(ms, ev.a, ev.b));
+        *ms /= kReps;
+        return FB_OK;
+    };
+    // Time each valid block size of one knob, keep the fastest in
+    // best.*field; then, for the passes with a block-order choice
+    // (`remap_pass` >= 0), time the XCD-contiguous order against the plain one.
     auto tune = [&](int knob, int pass, int order, unsigned fbk::Tuning::*field, unsigned unit,
-                    auto launch) -> fb_status {
+                    int remap_pass, auto launch) -> fb_status {
         float best_t = -1.0f;
         unsigned best_v = 0;
         for (unsigned c : candidates) {
             if (!fbk::tuning_valid(P, pass, order, c)) continue;
             fbk::Tuning t = best;
             t.*field = c / unit;
-            for (int k = 0; k < kWarm; ++k) FB_HIP_TRY(launch(t));
-            FB_HIP_TRY(hipEventRecord(ev.a, s));
-            for (int k = 0; k < kReps; ++k) FB_HIP_TRY(launch(t));
-            FB_HIP_TRY(hipEventRecord(ev.b, s));
-            FB_HIP_TRY(hipEventSynchronize(ev.b));
             float ms = 0.0f;
-            FB_HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-            ms /= kReps;
+            fb_status st2 = time(t, launch, &ms);
+            if (st2 != FB_OK) return st2;
             if (best_t < 0.0f || ms < best_t) {
                 best_t = ms;
                 best_v = c / unit;
             }
         }
-        if (best_v) {
-            best.*field = best_v;
-            best_ms[knob] = best_t;
+        if (!best_v) return FB_OK;
+        best.*field = best_v;
+        if (remap_pass >= 0) {
+            const unsigned bit = 1u << remap_pass;
+            fbk::Tuning t = best;
+            t.remap = best.remap ^ bit;
+            float ms = 0.0f;
+            fb_status st2 = time(t, launch, &ms);
+            if (st2 != FB_OK) return st2;
+            if (ms < best_t) {
+                best = t;
+                best_t = ms;
+            }
         }
+        best_ms[knob] = best_t;
         return FB_OK;
     };
     float4 *T = a->T.p, *E = a->E.p, *S = a->S.p;
@@ -131,29 +150,33 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
     // (The repeated launches rewrite the scratch set's LUTs, and the
     // indirect and multiple passes accumulate into E and S; the set is
     // discarded afterwards.)
-    st = tune(0, fbk::kPassSingle, 1, &fbk::Tuning::single, 1, [&](const fbk::Tuning& t) {
+    st = tune(0, fbk::kPassSingle, 1, &fbk::Tuning::single, 1, fbk::kPassSingle,
+              [&](const fbk::Tuning& t) {
         return fbk::launch_single_scattering(P, T, dR, dM, S, MR, t, s);
     });
     if (st == FB_OK)
-        st = tune(1, fbk::kPassDensity, 2, &fbk::Tuning::density2, 1, [&](const fbk::Tuning& t) {
+        st = tune(1, fbk::kPassDensity, 2, &fbk::Tuning::density2, 1, -1,
+                  [&](const fbk::Tuning& t) {
             return fbk::launch_scattering_density(P, 2, T, dR, dM, dMS, dE, J, t, s);
         });
     if (st == FB_OK)
-        st = tune(2, fbk::kPassDensity, 3, &fbk::Tuning::density, 1, [&](const fbk::Tuning& t) {
+        st = tune(2, fbk::kPassDensity, 3, &fbk::Tuning::density, 1, fbk::kPassDensity,
+                  [&](const fbk::Tuning& t) {
             return fbk::launch_scattering_density(P, 3, T, dR, dM, dMS, dE, J, t, s);
         });
     if (st == FB_OK)
-        st = tune(3, fbk::kPassIndirect, 2, &fbk::Tuning::indirect1, 64,
+        st = tune(3, fbk::kPassIndirect, 2, &fbk::Tuning::indirect1, 64, -1,
                   [&](const fbk::Tuning& t) {
                       return fbk::launch_indirect_irradiance(P, 1, dR, dM, dMS, dE, E, t, s);
                   });
     if (st == FB_OK)
-        st = tune(4, fbk::kPassIndirect, 3, &fbk::Tuning::indirect, 64,
+        st = tune(4, fbk::kPassIndirect, 3, &fbk::Tuning::indirect, 64, -1,
                   [&](const fbk::Tuning& t) {
                       return fbk::launch_indirect_irradiance(P, 2, dR, dM, dMS, dE, E, t, s);
                   });
     if (st == FB_OK)
-        st = tune(5, fbk::kPassMultiple, 2, &fbk::Tuning::multiple, 1, [&](const fbk::Tuning& t) {
+        st = tune(5, fbk::kPassMultiple, 2, &fbk::Tuning::multiple, 1, fbk::kPassMultiple,
+                  [&](const fbk::Tuning& t) {
             return fbk::launch_multiple_scattering(P, MR, J, dMS, S, t, s);
         });
     cleanup();
@@ -214,6 +237,13 @@ fb_status fb_builder_set_tuning(fb_builder* b, const fb_parameters* p, const fb_
         {t->multiple_scattering_threads, fbk::kPassMultiple, 2, 1, &fbk::Tuning::multiple},
     };
     fbk::Tuning tu;
+    const uint32_t remappable = (1u << fbk::kPassSingle) | (1u << fbk::kPassDensity) |
+                                (1u << fbk::kPassMultiple);
+    if (t->xcd_remap & ~remappable) {
+        set_error("xcd_remap bits other than single, density and multiple scattering");
+        return FB_EINVAL;
+    }
+    tu.remap = t->xcd_remap;
     for (const Knob& k : knobs) {
         if (k.v == 0) continue;  // default
         if (k.v > 256 || !fbk::tuning_valid(P, k.pass, k.order, k.v * k.unit)) {

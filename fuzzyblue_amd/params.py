@@ -9,6 +9,7 @@ atmosphere.  ``to_c()`` produces the C-ABI ``fb_parameters``.
 from __future__ import annotations
 
 import copy
+import dataclasses
 from dataclasses import dataclass, field
 from typing import List
 
@@ -113,6 +114,17 @@ class Parameters:
             self.scattering_mu_size,
             self.scattering_r_size,
         )
+
+    def to_dict(self) -> dict:
+        """Plain-data form (JSON fixtures)."""
+        return dataclasses.asdict(self)
+
+    @staticmethod
+    def from_dict(d: dict) -> "Parameters":
+        d = dict(d)
+        for k in ("rayleigh_density", "mie_density", "absorption_density"):
+            d[k] = DensityProfile([DensityProfileLayer(**l) for l in d[k]["layers"]])
+        return Parameters(**d)
 
     def footprint(self):
         """(resident, scratch) device bytes of one LUT set, as the library

@@ -126,8 +126,8 @@ void fb_parameters_extents(const fb_parameters *p, uint32_t transmittance[2],
                            uint32_t irradiance[2], uint32_t scattering[3]);
 
 /* Bytes of HBM a LUT set occupies: resident outputs (transmittance,
- * irradiance, scattering) and build-time scratch (five deltas).  Returns
- * resident + scratch. */
+ * irradiance, scattering) and build-time scratch (five deltas and the
+ * multiple-scattering row table).  Returns resident + scratch. */
 size_t fb_parameters_footprint(const fb_parameters *p, size_t *resident,
                                size_t *scratch);
 
@@ -177,6 +177,10 @@ typedef struct fb_tuning {
     uint32_t indirect_order1_waves;       /* 1, 2 or 4 wavefronts per block */
     uint32_t indirect_waves;              /* the same for orders >= 2 */
     uint32_t multiple_scattering_threads; /* 64, 128 or 256 per (r, mu) row */
+    uint32_t xcd_remap; /* bit p set (p = FB_PASS_SINGLE_SCATTERING,
+                           FB_PASS_SCATTERING_DENSITY or
+                           FB_PASS_MULTIPLE_SCATTERING): that pass gives each
+                           of the 8 XCDs a contiguous share of its blocks */
     float best_ms[6]; /* fb_builder_autotune: measured ms per launch of each
                          chosen shape, in field order; 0 otherwise */
 } fb_tuning;

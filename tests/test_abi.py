@@ -117,3 +117,17 @@ def test_ray_direction_unit_and_deterministic():
         d = ray_direction(2, 3, i)
         assert abs(float(np.linalg.norm(d.astype(np.float64))) - 1.0) < 1e-6
         assert np.array_equal(d, ray_direction(2, 3, i))
+
+
+def test_mixed_workload_matches_committed_fixture():
+    """workloads.mixed_batch(256, seed 4) is pinned by a committed JSON copy."""
+    import json
+    import os
+
+    from fuzzyblue_amd.workloads import mixed_batch
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "mixed256_seed4.json")
+    with open(path) as f:
+        fixed = [Parameters.from_dict(d) for d in json.load(f)]
+    assert fixed == mixed_batch(256, 4, precise=True)

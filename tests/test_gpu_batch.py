@@ -84,3 +84,40 @@ def test_batch_relaunch_refused_after_member_released(builder):
     atm.close()
     for pa in pend[1:]:
         pa.close()
+
+
+def test_mixed_half_members_vs_fp64_fixture(builder):
+    """Eight random atmospheres of the mixed workload's ranges (half
+    resolution, orders 2-8), built as one precision-mode batch, against their
+    FP64-oracle LUTs (tests/golden/make_golden_mixed.py)."""
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "mixed_half_fp64.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated")
+    g = np.load(path)
+    ps = [Parameters.from_dict(d) for d in json.loads(str(g["params"]))]
+    pend = Atmosphere.build_batch(builder, None, ps)
+    worst = {}
+    for i, (p, pa) in enumerate(zip(ps, pend)):
+        pa.wait()
+        a = pa.atmosphere()
+        got = {"T": a.download("transmittance")[..., :3],
+               "E": a.download("irradiance")[..., :3],
+               "S": a.download("scattering").reshape(-1, 4)[g["S_idx"]]}
+        errs = {}
+        for key in ("T", "E", "S"):
+            x = got[key].astype(np.float64)
+            y = g[f"{key}{i}"].astype(np.float64)
+            scale = np.abs(y).reshape(-1, y.shape[-1]).max(axis=0)
+            errs[key] = float((np.abs(x - y) / np.maximum(np.abs(y), 1e-6 * scale)).max())
+            worst[key] = max(worst.get(key, 0.0), errs[key])
+        print(f"\nmember {i} (order {p.order}, bottom {p.bottom_radius:.0f} km, thickness "
+              f"{p.top_radius - p.bottom_radius:.0f} km): {errs}", flush=True)
+    for pa in pend:
+        pa.close()
+    assert worst["T"] < 1e-5, worst
+    assert worst["E"] < 1e-3, worst
+    assert worst["S"] < 1e-3, worst

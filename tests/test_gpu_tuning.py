@@ -33,10 +33,13 @@ def test_every_block_shape_gives_identical_luts(precise):
     tuned, t = b.tuning(p)
     assert not tuned
     base = build(b, p)
-    for threads, waves in ((64, 1), (128, 2), (256, 4)):
+    remap_all = (1 << abi.FB_PASS_SINGLE_SCATTERING) | (1 << abi.FB_PASS_SCATTERING_DENSITY) \
+        | (1 << abi.FB_PASS_MULTIPLE_SCATTERING)
+    for threads, waves, remap in ((64, 1, 0), (128, 2, remap_all), (256, 4, remap_all)):
         b.set_tuning(p, single_scattering_threads=threads, density_order2_threads=threads,
                      density_threads=threads, indirect_order1_waves=waves,
-                     indirect_waves=waves, multiple_scattering_threads=threads)
+                     indirect_waves=waves, multiple_scattering_threads=threads,
+                     xcd_remap=remap)
         tuned, t = b.tuning(p)
         assert tuned and t["density_threads"] == threads and t["indirect_waves"] == waves
         same(base, build(b, p))
@@ -57,6 +60,8 @@ def test_set_tuning_rejects_invalid_shapes():
         b.set_tuning(p, multiple_scattering_threads=128)  # > 4 texels per thread
     with pytest.raises(FuzzyblueError):
         b.set_tuning(p, indirect_waves=8)
+    with pytest.raises(FuzzyblueError):
+        b.set_tuning(p, xcd_remap=1 << abi.FB_PASS_TRANSMITTANCE)
     b.set_tuning(p, multiple_scattering_threads=256)
     b.close()
 
