@@ -29,6 +29,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default);
+# streams sharing a queue run in FIFO order, so a render could wait behind a
+# whole precompute.  The serving run uses 6 streams: give each its own
+# queue.  Must be set before the HIP runtime initialises.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 from fuzzyblue_amd import (  # noqa: E402
     Atmosphere,
     Builder,

@@ -162,6 +162,7 @@ class fb_server_config(C.Structure):
         ("max_inflight", C.c_uint32),
         ("n_streams", C.c_uint32),
         ("keep_outputs", C.c_uint32),
+        ("precompute_share_pct", C.c_uint32),
     ]
 
 
@@ -174,6 +175,8 @@ class fb_request_info(C.Structure):
         ("t_submit", C.c_double),
         ("t_start", C.c_double),
         ("t_done", C.c_double),
+        ("gpu_ms", C.c_float),
+        ("gpu_wait_ms", C.c_float),
     ]
 
 
@@ -186,4 +189,6 @@ class fb_server_stats(C.Structure):
         ("hbm_in_use", C.c_size_t),
         ("hbm_peak", C.c_size_t),
         ("resident_sets", C.c_uint32),
+        ("step_max_ms", C.c_float * 3),
+        ("slow_steps", C.c_uint64),
     ]

@@ -10,6 +10,8 @@
 // order loop runs to the largest order in the batch; an atmosphere of order
 // k contributes no tiles to the launches of orders above k.  The whole
 // schedule is captured once as a hipGraph and replayed.
+#include <string.h>
+
 #include <algorithm>
 #include <memory>
 #include <vector>
@@ -27,6 +29,9 @@ struct BatchLaunch {
     unsigned ntiles = 0;
     unsigned threads = 0;
     size_t lds = 0;
+    std::vector<uint32_t> host_tiles;  // until uploaded
+    size_t offset = 0;                 // of d_tiles in the plan's device block
+    unsigned grid = 0;                 // persistent grid size (0: one block per tile)
 };
 
 // Shared state of one batch: device arrays, the launch list and the graph.
@@ -41,20 +46,27 @@ struct BatchPlan {
     fbk::DevParams* d_params = nullptr;
     fbk::LutPtrs* d_luts = nullptr;
     std::vector<BatchLaunch> launches;
+    // Percentage of the device's resident-block capacity each launch may
+    // hold (100: one block per tile).  Below 100 every launch is persistent,
+    // leaving the remaining slots on every CU to other streams.
+    unsigned share_pct = 100;
     hipGraphExec_t graph = nullptr;
+    // One block from the builder's pool holds the parameter and LUT-pointer
+    // arrays and every tile list: no hipMalloc / hipFree (which synchronises
+    // the whole device) while other work runs.
+    void* block = nullptr;
+    size_t block_bytes = 0;
+    std::vector<uint8_t> staging;  // host image of `block`, alive until the plan dies
 
     ~BatchPlan() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        for (auto& l : launches)
-            if (l.d_tiles) (void)hipFree(l.d_tiles);
-        if (d_params) (void)hipFree(d_params);
-        if (d_luts) (void)hipFree(d_luts);
+        if (block) builder->pool->release(block, block_bytes);
     }
 
     fb_status enqueue(hipStream_t s) const {
         for (const auto& l : launches)
             FB_HIP_TRY(fbk::launch_batch(l.pass, l.order, precise, d_params, d_luts, l.d_tiles,
-                                         l.ntiles, l.threads, l.lds, s));
+                                         l.ntiles, l.threads, l.lds, l.grid, s));
         return FB_OK;
     }
 };
@@ -113,18 +125,62 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
         return FB_EINVAL;
     }
     l.ntiles = unsigned(tiles.size());
-    FB_HIP_TRY(hipMalloc(&l.d_tiles, tiles.size() * sizeof(uint32_t)));
-    plan.launches.push_back(l);  // owned by the plan from here on
-    FB_HIP_TRY(hipMemcpy(plan.launches.back().d_tiles, tiles.data(),
-                         tiles.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (plan.share_pct < 100) {
+        const unsigned resident =
+            fbk::batch_resident_blocks(pass, order, plan.precise, l.threads, l.lds);
+        if (resident) l.grid = std::max(1u, resident * plan.share_pct / 100);
+    }
+    l.host_tiles = std::move(tiles);
+    plan.launches.push_back(std::move(l));
+    return FB_OK;
+}
+
+// Place the parameter / LUT-pointer arrays and the tile lists in one pool
+// block and upload them with one asynchronous copy on the launch stream,
+// ahead of the batch's first launch: the host waits for nothing (a
+// synchronous copy, or a wait on another stream, can sit behind unrelated
+// work that shares the stream's hardware queue).
+fb_status upload_plan(fb_builder* b, BatchPlan& plan, const std::vector<fbk::DevParams>& dps,
+                      const std::vector<fbk::LutPtrs>& luts, hipStream_t s) {
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t n = dps.size();
+    const size_t off_luts = align(n * sizeof(fbk::DevParams));
+    size_t off = off_luts + align(n * sizeof(fbk::LutPtrs));
+    for (auto& l : plan.launches) {
+        l.offset = off;
+        off += align(l.host_tiles.size() * sizeof(uint32_t));
+    }
+    constexpr size_t kGrain = size_t(256) << 10;  // size classes the pool can reuse
+    plan.block_bytes = (off + kGrain - 1) / kGrain * kGrain;
+    fb_status st = b->pool->alloc(plan.block_bytes, &plan.block);
+    if (st != FB_OK) {
+        plan.block = nullptr;
+        return st;
+    }
+    std::vector<uint8_t>& staging = plan.staging;
+    staging.assign(off, 0);
+    memcpy(staging.data(), dps.data(), n * sizeof(fbk::DevParams));
+    memcpy(staging.data() + off_luts, luts.data(), n * sizeof(fbk::LutPtrs));
+    for (const auto& l : plan.launches)
+        memcpy(staging.data() + l.offset, l.host_tiles.data(),
+               l.host_tiles.size() * sizeof(uint32_t));
+    uint8_t* base = static_cast<uint8_t*>(plan.block);
+    FB_HIP_TRY(hipMemcpyAsync(base, staging.data(), off, hipMemcpyHostToDevice, s));
+    plan.d_params = reinterpret_cast<fbk::DevParams*>(base);
+    plan.d_luts = reinterpret_cast<fbk::LutPtrs*>(base + off_luts);
+    for (auto& l : plan.launches) {
+        l.d_tiles = reinterpret_cast<uint32_t*>(base + l.offset);
+        l.host_tiles = std::vector<uint32_t>();
+    }
     return FB_OK;
 }
 
 // Build (and launch) one group: same precision flags, at most 256 members.
 fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_parameters>& ps,
-                      fb_pending_atmosphere** out) {
+                      fb_pending_atmosphere** out, unsigned share_pct, bool direct) {
     auto plan = std::make_shared<BatchPlan>();
     plan->builder = b;
+    plan->share_pct = share_pct;
     plan->precise = (ps[0].flags & FB_FLAG_PRECISE) != 0;
     const size_t n = ps.size();
     std::vector<fbk::DevParams> dps(n);
@@ -140,20 +196,14 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
         luts[i] = fbk::LutPtrs{a->T.p,  a->E.p,   a->S.p, pa->dE.p, pa->dR.p,
                                pa->dM.p, pa->dMS.p, pa->J.p, pa->MR.p};
     }
+    bool uploaded = false;  // an asynchronous copy into plan->block is enqueued on s
     auto fail = [&](fb_status code) {
+        if (uploaded) (void)hipStreamSynchronize(s);  // before block and staging go away
         for (auto* pa : plan->members) fb_pending_destroy(pa);
         plan->members.clear();
         return code;
     };
     if (st != FB_OK) return fail(st);
-    hipError_t e = hipMalloc(&plan->d_params, n * sizeof(fbk::DevParams));
-    if (e == hipSuccess) e = hipMalloc(&plan->d_luts, n * sizeof(fbk::LutPtrs));
-    if (e == hipSuccess)
-        e = hipMemcpy(plan->d_params, dps.data(), n * sizeof(fbk::DevParams),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(plan->d_luts, luts.data(), n * sizeof(fbk::LutPtrs), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return fail(hip_fail(e, "batch parameter upload"));
 
     const std::vector<int> order_idx = tile_order(ps);
     int max_order = 1;
@@ -167,6 +217,38 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
         for (int pass : {fbk::kPassDensity, fbk::kPassIndirect, fbk::kPassMultiple})
             if ((st = add_launch(*plan, ps, dps, tus, order_idx, pass, order)) != FB_OK)
                 return fail(st);
+    st = upload_plan(b, *plan, dps, luts, s);
+    uploaded = plan->block != nullptr;
+    if (st != FB_OK) return fail(st);
+
+    hipError_t e = hipSuccess;
+    if (direct) {
+        // Launch pass by pass and complete each member right after the last
+        // launch it takes part in: a small or low-order atmosphere is
+        // resident long before the batch's largest member.  (No graph: the
+        // batch runs once.)
+        std::vector<size_t> last(n, 0);
+        for (size_t li = 0; li < plan->launches.size(); ++li)
+            for (size_t i = 0; i < n; ++i)
+                if (plan->launches[li].pass < fbk::kPassDensity ||
+                    int(ps[i].order) >= plan->launches[li].order)
+                    last[i] = li;
+        for (size_t li = 0; li < plan->launches.size(); ++li) {
+            const BatchLaunch& l = plan->launches[li];
+            e = fbk::launch_batch(l.pass, l.order, plan->precise, plan->d_params, plan->d_luts,
+                                  l.d_tiles, l.ntiles, l.threads, l.lds, l.grid, s);
+            for (size_t i = 0; i < n && e == hipSuccess; ++i) {
+                if (last[i] != li) continue;
+                e = hipEventRecord(plan->members[i]->done, s);
+                if (e == hipSuccess) e = hipEventRecord(plan->members[i]->atm->ready, s);
+            }
+            if (e != hipSuccess) return fail(hip_fail(e, "batch launch"));
+        }
+        plan->n_total = plan->members.size();
+        for (auto* pa : plan->members) pa->batch = plan;
+        for (size_t i = 0; i < n; ++i) out[i] = plan->members[i];
+        return FB_OK;
+    }
 
     // Capture the schedule once; replays reuse it (fb_pending_relaunch).
     {
@@ -213,6 +295,10 @@ namespace fbrt {
 fb_status batch_relaunch(fb_pending_atmosphere* pa, hipStream_t s) {
     auto plan = std::static_pointer_cast<BatchPlan>(pa->batch);
     std::lock_guard<std::mutex> g(plan->mu);
+    if (!plan->graph) {
+        set_error("this batch was launched once without a graph and cannot be relaunched");
+        return FB_EINVAL;
+    }
     if (plan->members.size() != plan->n_total) {
         set_error("batch has released members; it can no longer be relaunched");
         return FB_EBUSY;
@@ -238,9 +324,9 @@ void batch_detach(fb_pending_atmosphere* pa) {
 }
 }  // namespace fbrt
 
-extern "C" fb_status fb_atmosphere_build_batch(fb_builder* b, fb_stream stream,
-                                               const fb_parameters* params, size_t n,
-                                               fb_pending_atmosphere** out) {
+namespace fbrt {
+fb_status build_batch(fb_builder* b, hipStream_t s, const fb_parameters* params, size_t n,
+                      fb_pending_atmosphere** out, unsigned share_pct, bool direct) {
     if (!b || (!params && n) || (!out && n)) {
         set_error("NULL argument");
         return FB_EINVAL;
@@ -251,7 +337,6 @@ extern "C" fb_status fb_atmosphere_build_batch(fb_builder* b, fb_stream stream,
         out[i] = nullptr;
     }
     FB_HIP_TRY(hipSetDevice(b->device));
-    hipStream_t s = static_cast<hipStream_t>(stream);
     // Groups: one per precision mode, at most 256 atmospheres each (the tile
     // word's 8-bit atmosphere slot).
     std::vector<size_t> built;
@@ -264,7 +349,7 @@ extern "C" fb_status fb_atmosphere_build_batch(fb_builder* b, fb_stream stream,
             std::vector<fb_parameters> ps;
             for (size_t k = g0; k < g1; ++k) ps.push_back(params[idx[k]]);
             std::vector<fb_pending_atmosphere*> outs(ps.size(), nullptr);
-            fb_status st = build_group(b, s, ps, outs.data());
+            fb_status st = build_group(b, s, ps, outs.data(), share_pct, direct);
             if (st != FB_OK) {
                 for (size_t i : built) {
                     fb_pending_destroy(out[i]);
@@ -279,4 +364,11 @@ extern "C" fb_status fb_atmosphere_build_batch(fb_builder* b, fb_stream stream,
         }
     }
     return FB_OK;
+}
+}  // namespace fbrt
+
+extern "C" fb_status fb_atmosphere_build_batch(fb_builder* b, fb_stream stream,
+                                               const fb_parameters* params, size_t n,
+                                               fb_pending_atmosphere** out) {
+    return build_batch(b, static_cast<hipStream_t>(stream), params, n, out, 100, false);
 }

@@ -80,9 +80,13 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
 // `order` is the scattering order of the launch (density n, indirect n-1's
 // caller order n, multiple n); threads / lds are the maxima over the tiles'
 // atmospheres.
+// With grid_blocks in (0, ntiles) the launch is persistent: grid_blocks
+// blocks walk the tile list (0 = one block per tile).
 hipError_t launch_batch(int pass, int order, bool precise, const DevParams* d_params,
                         const LutPtrs* d_luts, const uint32_t* d_tiles, unsigned ntiles,
-                        unsigned threads, size_t lds, hipStream_t s);
+                        unsigned threads, size_t lds, unsigned grid_blocks, hipStream_t s);
+// Blocks of a batched launch of `pass` resident on the device at once.
+unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threads, size_t lds);
 
 hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s);
 hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,

@@ -948,10 +948,16 @@ __global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __re
 template <typename G, bool COMP, int LANES>
 __global__ __launch_bounds__(256) void kb_transmittance(const DevParams* __restrict__ Ps,
                                                         const LutPtrs* __restrict__ L,
-                                                        const uint32_t* __restrict__ tiles) {
-    int atm, local;
-    decode_tile(tiles[blockIdx.x], atm, local);
-    transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, L[atm].T);
+                                                        const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    // Persistent when the grid is smaller than the tile list: the block
+    // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
+    // tile's LDS reads apart from the next tile's writes.
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, L[atm].T);
+    }
 }
 
 template <typename G>
@@ -962,12 +968,18 @@ __global__ __launch_bounds__(256) void k_direct_irradiance(DevParams P, const fl
 template <typename G>
 __global__ __launch_bounds__(256) void kb_direct_irradiance(const DevParams* __restrict__ Ps,
                                                             const LutPtrs* __restrict__ L,
-                                                            const uint32_t* __restrict__ tiles) {
-    int atm, local;
-    decode_tile(tiles[blockIdx.x], atm, local);
-    const DevParams& P = Ps[atm];
-    const int gx = (P.i_mus + 63) / 64;
-    direct_irradiance_block<G>(P, local % gx, local / gx, 0, L[atm].T, L[atm].dE);
+                                                            const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    // Persistent when the grid is smaller than the tile list: the block
+    // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
+    // tile's LDS reads apart from the next tile's writes.
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const DevParams& P = Ps[atm];
+        const int gx = (P.i_mus + 63) / 64;
+        direct_irradiance_block<G>(P, local % gx, local / gx, 0, L[atm].T, L[atm].dE);
+    }
 }
 
 // Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has
@@ -998,13 +1010,19 @@ __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const fl
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
                                                             const LutPtrs* __restrict__ L,
-                                                            const uint32_t* __restrict__ tiles) {
-    int atm, local;
-    decode_tile(tiles[blockIdx.x], atm, local);
-    const DevParams& P = Ps[atm];
-    const LutPtrs& l = L[atm];
-    single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.dR, l.dM, l.S,
-                                     l.MR);
+                                                            const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    // Persistent when the grid is smaller than the tile list: the block
+    // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
+    // tile's LDS reads apart from the next tile's writes.
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const DevParams& P = Ps[atm];
+        const LutPtrs& l = L[atm];
+        single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.dR, l.dM, l.S,
+                                         l.MR);
+    }
 }
 
 template <typename G, bool SINGLE>
@@ -1019,14 +1037,20 @@ __global__ __launch_bounds__(256, 8) void k_scattering_density(
 template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams* __restrict__ Ps,
                                                                 const LutPtrs* __restrict__ L,
-                                                                const uint32_t* __restrict__ tiles) {
-    int atm, local;
-    decode_tile(tiles[blockIdx.x], atm, local);
-    const DevParams& P = Ps[atm];
-    const LutPtrs& l = L[atm];
-    const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
-    scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
-                                        local / (gx * P.s_mus), l.T, l.dR, l.dM, l.dMS, l.dE, l.J);
+                                                                const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    // Persistent when the grid is smaller than the tile list: the block
+    // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
+    // tile's LDS reads apart from the next tile's writes.
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const DevParams& P = Ps[atm];
+        const LutPtrs& l = L[atm];
+        const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
+        scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
+                                            local / (gx * P.s_mus), l.T, l.dR, l.dM, l.dMS, l.dE, l.J);
+    }
 }
 
 template <typename G, bool SINGLE>
@@ -1038,11 +1062,17 @@ __global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
 template <typename G, bool SINGLE>
 __global__ __launch_bounds__(64 * kIndWaves) void kb_indirect_irradiance(
     const DevParams* __restrict__ Ps, const LutPtrs* __restrict__ L,
-    const uint32_t* __restrict__ tiles) {
-    int atm, local;
-    decode_tile(tiles[blockIdx.x], atm, local);
-    const LutPtrs& l = L[atm];
-    indirect_irradiance_block<G, SINGLE>(Ps[atm], local, 0, 0, l.dR, l.dM, l.dMS, l.dE, l.E);
+    const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    // Persistent when the grid is smaller than the tile list: the block
+    // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
+    // tile's LDS reads apart from the next tile's writes.
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const LutPtrs& l = L[atm];
+        indirect_irradiance_block<G, SINGLE>(Ps[atm], local, 0, 0, l.dR, l.dM, l.dMS, l.dE, l.E);
+    }
 }
 
 template <typename G, bool COMP, int TPT>
@@ -1058,13 +1088,19 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
                                                               const LutPtrs* __restrict__ L,
-                                                              const uint32_t* __restrict__ tiles) {
-    int atm, local;
-    decode_tile(tiles[blockIdx.x], atm, local);
-    const DevParams& P = Ps[atm];
-    const LutPtrs& l = L[atm];
-    multiple_scattering_block<G, COMP, kMsTexelsPerThread>(P, local % P.s_mu, local / P.s_mu, 0,
-                                                          l.MR, l.J, l.dMS, l.S);
+                                                              const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    // Persistent when the grid is smaller than the tile list: the block
+    // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
+    // tile's LDS reads apart from the next tile's writes.
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const DevParams& P = Ps[atm];
+        const LutPtrs& l = L[atm];
+        multiple_scattering_block<G, COMP, kMsTexelsPerThread>(P, local % P.s_mu, local / P.s_mu, 0,
+                                                              l.MR, l.J, l.dMS, l.S);
+    }
 }
 
 // ================================================================== clear
@@ -1075,12 +1111,18 @@ __global__ void k_fill_zero(float4* __restrict__ p, size_t n) {
 // Clear every atmosphere's irradiance LUT (src/precompute.rs:1802-1816).
 __global__ void kb_clear_irradiance(const DevParams* __restrict__ Ps,
                                     const LutPtrs* __restrict__ L,
-                                    const uint32_t* __restrict__ tiles) {
-    int atm, local;
-    decode_tile(tiles[blockIdx.x], atm, local);
-    const size_t n = size_t(Ps[atm].i_mus) * Ps[atm].i_r;
-    const size_t i = size_t(local) * blockDim.x + threadIdx.x;
-    if (i < n) L[atm].E[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                                    const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    // Persistent when the grid is smaller than the tile list: the block
+    // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
+    // tile's LDS reads apart from the next tile's writes.
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const size_t n = size_t(Ps[atm].i_mus) * Ps[atm].i_r;
+        const size_t i = size_t(local) * blockDim.x + threadIdx.x;
+        if (i < n) L[atm].E[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
 }
 
 // ========================================================== launch wrappers
@@ -1266,62 +1308,111 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
 
 hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
                         const LutPtrs* L, const uint32_t* tiles, unsigned ntiles,
-                        unsigned threads, size_t lds, hipStream_t s) {
+                        unsigned threads, size_t lds, unsigned grid_blocks, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
-    const dim3 grid(ntiles), block(threads);
+    const dim3 grid(grid_blocks && grid_blocks < ntiles ? grid_blocks : ntiles), block(threads);
     const bool single = (order - 1) == 1;
     switch (pass) {
         case kPassTransmittance:
             if (precise)
-                kb_transmittance<double, true, 8><<<grid, block, 0, s>>>(Ps, L, tiles);
+                kb_transmittance<double, true, 8><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_transmittance<float, false, 1><<<grid, block, 0, s>>>(Ps, L, tiles);
+                kb_transmittance<float, false, 1><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassDirect:
             if (precise)
-                kb_direct_irradiance<double><<<grid, block, 0, s>>>(Ps, L, tiles);
+                kb_direct_irradiance<double><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_direct_irradiance<float><<<grid, block, 0, s>>>(Ps, L, tiles);
+                kb_direct_irradiance<float><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassSingle:
             if (precise)
-                kb_single_scattering<double, false><<<grid, block, 0, s>>>(Ps, L, tiles);
+                kb_single_scattering<double, false><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_single_scattering<float, false><<<grid, block, 0, s>>>(Ps, L, tiles);
+                kb_single_scattering<float, false><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassClear:
-            kb_clear_irradiance<<<grid, block, 0, s>>>(Ps, L, tiles);
+            kb_clear_irradiance<<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassDensity:
             if (precise && single)
-                kb_scattering_density<double, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_scattering_density<double, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (precise)
-                kb_scattering_density<double, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_scattering_density<double, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (single)
-                kb_scattering_density<float, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_scattering_density<float, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_scattering_density<float, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_scattering_density<float, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassIndirect:
             if (precise && single)
-                kb_indirect_irradiance<double, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_indirect_irradiance<double, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (precise)
-                kb_indirect_irradiance<double, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_indirect_irradiance<double, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (single)
-                kb_indirect_irradiance<float, true><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_indirect_irradiance<float, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_indirect_irradiance<float, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_indirect_irradiance<float, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassMultiple:
             if (precise)
-                kb_multiple_scattering<double, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_multiple_scattering<double, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_multiple_scattering<float, false><<<grid, block, lds, s>>>(Ps, L, tiles);
+                kb_multiple_scattering<float, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             break;
         default:
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Blocks of one batched launch that fit on the device at once (per-CU
+// occupancy of the launch's kernel x CUs); 0 if it cannot be determined.
+unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threads, size_t lds) {
+    const bool single = (order - 1) == 1;
+    const void* fn = nullptr;
+    switch (pass) {
+        case kPassTransmittance:
+            fn = precise ? (const void*)kb_transmittance<double, true, 8>
+                         : (const void*)kb_transmittance<float, false, 1>;
+            break;
+        case kPassDirect:
+            fn = precise ? (const void*)kb_direct_irradiance<double>
+                         : (const void*)kb_direct_irradiance<float>;
+            break;
+        case kPassSingle:
+            fn = precise ? (const void*)kb_single_scattering<double, false>
+                         : (const void*)kb_single_scattering<float, false>;
+            break;
+        case kPassClear:
+            fn = (const void*)kb_clear_irradiance;
+            break;
+        case kPassDensity:
+            fn = precise ? (single ? (const void*)kb_scattering_density<double, true>
+                                   : (const void*)kb_scattering_density<double, false>)
+                         : (single ? (const void*)kb_scattering_density<float, true>
+                                   : (const void*)kb_scattering_density<float, false>);
+            break;
+        case kPassIndirect:
+            fn = precise ? (single ? (const void*)kb_indirect_irradiance<double, true>
+                                   : (const void*)kb_indirect_irradiance<double, false>)
+                         : (single ? (const void*)kb_indirect_irradiance<float, true>
+                                   : (const void*)kb_indirect_irradiance<float, false>);
+            break;
+        case kPassMultiple:
+            fn = precise ? (const void*)kb_multiple_scattering<double, false>
+                         : (const void*)kb_multiple_scattering<float, false>;
+            break;
+    }
+    int per_cu = 0, dev = 0, cus = 0;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, int(threads), lds) !=
+                   hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return unsigned(per_cu) * unsigned(cus);
 }
 
 hipError_t launch_fill_zero(float4* p, size_t n, hipStream_t s) {

@@ -126,7 +126,13 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
 fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_a,
                      fb_pending_atmosphere** out_pa);
 void free_pending_scratch(fb_pending_atmosphere* pa);
-// Batches (batch.hip).
+// Batches (batch.hip).  share_pct < 100: persistent launches holding at
+// most that share of the device's resident-block capacity.  direct: launch
+// once without a graph and complete each member after its own last pass
+// (no relaunch); otherwise capture a graph and complete all members at the
+// end.
+fb_status build_batch(fb_builder* b, hipStream_t s, const fb_parameters* params, size_t n,
+                      fb_pending_atmosphere** out, unsigned share_pct, bool direct);
 fb_status batch_relaunch(fb_pending_atmosphere* pa, hipStream_t s);
 void batch_detach(fb_pending_atmosphere* pa);
 void free_atmosphere_buffers(fb_atmosphere* a);

@@ -13,8 +13,11 @@
 //      sets that no render pins — and launches every admitted group of the
 //      same precision as ONE batch (one launch per pass over all of them).
 // Renders have a device stream of their own and precomputes rotate over the
-// others, so renders run beside long precomputes instead of behind them.  Memory admission counts resident LUT sets, in-flight scratch
-// and render outputs against the budget.
+// others.  With precompute_share_pct below 100 every precompute launch is
+// persistent and sized to that share of the device's resident-block slots,
+// so a render arriving mid-precompute finds free slots on every CU instead
+// of waiting for the precompute's blocks to drain.  Memory admission counts
+// resident LUT sets, in-flight scratch and render outputs against the budget.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -51,7 +54,7 @@ struct Req {
     double t_submit = 0.0, t_start = 0.0;
     // in flight
     fb_pending_atmosphere* pending = nullptr;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr, ev0 = nullptr, ev_adm = nullptr;
     float4* out = nullptr;
     size_t out_bytes = 0;
     size_t reserved = 0;
@@ -69,6 +72,7 @@ struct fb_server {
     fb_builder* b = nullptr;
     fb_server_config cfg{};
     std::vector<hipStream_t> streams;
+    hipStream_t clock_stream = nullptr;  // idle stream: admission timestamps
     size_t next_stream = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_idle;
@@ -98,8 +102,10 @@ struct fb_server {
         peak = std::max(peak, in_use);
     }
 
-    void finish(Req* r, fb_status st) {
+    void finish(Req* r, fb_status st, float gpu_ms = 0.0f, float gpu_wait_ms = 0.0f) {
         fb_request_info info{};
+        info.gpu_ms = gpu_ms;
+        info.gpu_wait_ms = gpu_wait_ms;
         info.id = r->id;
         info.kind = r->kind;
         info.status = int32_t(st);
@@ -168,7 +174,19 @@ struct fb_server {
                     still.push_back(r);
                     continue;
                 }
+                float gpu_ms = 0.0f, gpu_wait_ms = 0.0f;
+                if (e == hipSuccess && hipEventElapsedTime(&gpu_ms, r->ev0, r->ev) != hipSuccess) {
+                    (void)hipGetLastError();
+                    gpu_ms = 0.0f;
+                }
+                if (e == hipSuccess &&
+                    hipEventElapsedTime(&gpu_wait_ms, r->ev_adm, r->ev0) != hipSuccess) {
+                    (void)hipGetLastError();
+                    gpu_wait_ms = 0.0f;
+                }
                 (void)hipEventDestroy(r->ev);
+                (void)hipEventDestroy(r->ev0);
+                (void)hipEventDestroy(r->ev_adm);
                 pins[r->lut_id]--;
                 if (cfg.keep_outputs && e == hipSuccess) {
                     outputs[r->id] = {r->out, r->out_bytes};
@@ -176,7 +194,7 @@ struct fb_server {
                     b->pool->release(r->out, r->out_bytes);
                     account(0, r->out_bytes);
                 }
-                finish(r, e == hipSuccess ? FB_OK : FB_EHIP);
+                finish(r, e == hipSuccess ? FB_OK : FB_EHIP, gpu_ms, gpu_wait_ms);
             }
         }
         inflight.swap(still);
@@ -224,8 +242,12 @@ struct fb_server {
                 continue;
             }
             hipStream_t s = render_stream();
-            hipError_t e = hipEventCreateWithFlags(&r->ev, hipEventDisableTiming);
+            hipError_t e = hipEventCreate(&r->ev);
+            if (e == hipSuccess) e = hipEventCreate(&r->ev0);
+            if (e == hipSuccess) e = hipEventCreate(&r->ev_adm);
+            if (e == hipSuccess) e = hipEventRecord(r->ev_adm, clock_stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, atm->ready, 0);
+            if (e == hipSuccess) e = hipEventRecord(r->ev0, s);
             if (e == hipSuccess)
                 e = fbk::launch_render_frame(atm->dev, atm->T.p, atm->S.p, r->alt, r->zen,
                                              r->w * r->h, r->seed, static_cast<float4*>(out), s);
@@ -234,6 +256,8 @@ struct fb_server {
             if (e != hipSuccess) {
                 (void)hipStreamSynchronize(s);
                 if (r->ev) (void)hipEventDestroy(r->ev);
+                if (r->ev0) (void)hipEventDestroy(r->ev0);
+                if (r->ev_adm) (void)hipEventDestroy(r->ev_adm);
                 b->pool->release(out, bytes);
                 finish(r, hip_fail(e, "render launch"));
                 continue;
@@ -283,9 +307,11 @@ struct fb_server {
             for (Req* r : g) ps.push_back(r->params);
             std::vector<fb_pending_atmosphere*> outs(g.size(), nullptr);
             hipStream_t s = precompute_stream();
-            fb_status st = g.size() == 1
+            const unsigned share = cfg.precompute_share_pct;
+            fb_status st = (g.size() == 1 && (share == 0 || share >= 100))
                                ? fb_atmosphere_build(b, s, &ps[0], &outs[0])
-                               : fb_atmosphere_build_batch(b, s, ps.data(), ps.size(), outs.data());
+                               : build_batch(b, s, ps.data(), ps.size(), outs.data(),
+                                             share == 0 || share >= 100 ? 100u : share, true);
             double t = now_s();
             for (size_t i = 0; i < g.size(); ++i) {
                 Req* r = g[i];
@@ -312,9 +338,18 @@ struct fb_server {
                 cv_work.wait(lk, [&] { return stop || !waiting.empty(); });
                 continue;
             }
+            double t0 = now_s();
             retire();
+            double t1 = now_s();
             admit_renders();
+            double t2 = now_s();
             admit_precomputes();
+            double t3 = now_s();
+            const double steps[3] = {t1 - t0, t2 - t1, t3 - t2};
+            for (int k = 0; k < 3; ++k) {
+                stats.step_max_ms[k] = std::max(stats.step_max_ms[k], float(1e3 * steps[k]));
+                if (steps[k] > 2e-3) stats.slow_steps++;
+            }
             if (inflight.empty() && !waiting.empty()) {
                 // Nothing runs, so nothing will free memory: what is still
                 // waiting cannot fit the budget.
@@ -355,9 +390,11 @@ fb_status fb_server_create(fb_builder* b, const fb_server_config* cfg, fb_server
         e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
         if (e == hipSuccess) s->streams.push_back(st);
     }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->clock_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         fb_status st = hip_fail(e, "server streams");
         for (auto st2 : s->streams) (void)hipStreamDestroy(st2);
+        if (s->clock_stream) (void)hipStreamDestroy(s->clock_stream);
         builder_release(b);
         delete s;
         return st;
@@ -379,6 +416,7 @@ void fb_server_destroy(fb_server* s) {
     for (auto& kv : s->resident) fb_atmosphere_destroy(kv.second.atm);
     for (auto& kv : s->outputs) s->b->pool->release(kv.second.first, kv.second.second);
     for (auto st : s->streams) (void)hipStreamDestroy(st);
+    (void)hipStreamDestroy(s->clock_stream);
     builder_release(s->b);
     delete s;
 }

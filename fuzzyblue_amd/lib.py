@@ -41,6 +41,12 @@ def load():
             abi.FB_ENODEV,
             f"{LIB_PATH} is missing: build it with `python -m fuzzyblue_amd.build`",
         )
+    # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process
+    # (default 4); streams beyond that share a queue and run in FIFO order
+    # with each other.  The serving loop's render stream must not queue
+    # behind a precompute stream, so ask for one queue per stream unless the
+    # caller chose a value (read when the HIP runtime initialises).
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     lib = C.CDLL(LIB_PATH)
     P = C.POINTER(abi.fb_parameters)
     V = C.c_void_p

@@ -22,9 +22,9 @@ from .workloads import Request
 
 class Server:
     def __init__(self, builder: Builder, hbm_budget_bytes: int, max_inflight: int = 8,
-                 n_streams: int = 4, keep_outputs: bool = False):
+                 n_streams: int = 4, keep_outputs: bool = False, precompute_share_pct: int = 0):
         cfg = abi.fb_server_config(int(hbm_budget_bytes), int(max_inflight), int(n_streams),
-                                   1 if keep_outputs else 0)
+                                   1 if keep_outputs else 0, int(precompute_share_pct))
         self.handle = C.c_void_p()
         check(load().fb_server_create(builder.handle, C.byref(cfg), C.byref(self.handle)))
         self.builder = builder
@@ -127,12 +127,29 @@ def replay(server: Server, requests: List[Request], warm_luts: Optional[List[int
         failed = sum(1 for i in infos if kinds.get(i.id) == kind and i.status != abi.FB_OK)
         entry = {"completed": int(lat.size), "failed": int(failed)}
         if lat.size:
+            ok = [i for i in infos if kinds.get(i.id) == kind and i.status == abi.FB_OK]
+            wait = np.array([1e3 * (i.t_start - i.t_submit) for i in ok])
+            run = np.array([1e3 * (i.t_done - i.t_start) for i in ok])
             entry.update(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                         mean_ms=float(lat.mean()), per_s=float(lat.size / wall))
+                         mean_ms=float(lat.mean()), per_s=float(lat.size / wall),
+                         # split: queued (submit -> admitted) and on device
+                         # (admitted -> completion observed)
+                         queue_p99_ms=float(np.percentile(wait, 99)),
+                         device_p99_ms=float(np.percentile(run, 99)))
+            if kind == "render":
+                gpu = np.array([i.gpu_ms for i in ok])
+                gwait = np.array([i.gpu_wait_ms for i in ok])
+                entry.update(gpu_p50_ms=float(np.percentile(gpu, 50)),
+                             gpu_p99_ms=float(np.percentile(gpu, 99)),
+                             gpu_wait_p99_ms=float(np.percentile(gwait, 99)))
         out[kind] = entry
     done_px = sum(pixels[i.id] for i in infos if i.id in pixels and i.status == abi.FB_OK)
     out["render_mpixel_s"] = done_px / wall / 1e6
     st = server.stats()
     out["evictions"] = int(st.evictions)
     out["hbm_peak_gb"] = st.hbm_peak / 1e9
+    out["worker_step_max_ms"] = {"retire": float(st.step_max_ms[0]),
+                                 "admit_renders": float(st.step_max_ms[1]),
+                                 "admit_precomputes": float(st.step_max_ms[2])}
+    out["worker_slow_steps"] = int(st.slow_steps)
     return out

@@ -375,6 +375,11 @@ typedef struct fb_server_config {
     uint32_t n_streams;        /* device streams the server schedules on */
     uint32_t keep_outputs;     /* nonzero: keep each render's radiance until
                                   fb_server_render_result fetches it */
+    uint32_t precompute_share_pct; /* 1..99: every precompute launch is
+                                  persistent and holds at most this share of
+                                  the device's resident-block slots, so renders
+                                  find free slots on every CU; 0 or >= 100: one
+                                  block per tile (full occupancy) */
 } fb_server_config;
 
 typedef enum fb_request_kind {
@@ -390,6 +395,12 @@ typedef struct fb_request_info {
     double t_submit;     /* seconds, host monotonic clock */
     double t_start;      /* admitted onto the device */
     double t_done;       /* completion observed by the server */
+    float gpu_ms;        /* render: GPU time from the launch reaching the
+                            render stream to the kernel's end (queueing for
+                            CUs included); precompute: 0 */
+    float gpu_wait_ms;   /* render: GPU time from admission (a timestamp on an
+                            idle stream) to the launch reaching the render
+                            stream; precompute: 0 */
 } fb_request_info;
 
 fb_status fb_server_create(fb_builder *b, const fb_server_config *cfg,
@@ -419,6 +430,11 @@ typedef struct fb_server_stats {
     uint64_t submitted, completed, rejected, evictions;
     size_t hbm_in_use, hbm_peak;
     uint32_t resident_sets;
+    /* Longest single step of the worker loop, in ms: retiring finished work,
+     * admitting renders, admitting precomputes (host time, HIP calls
+     * included); and how many steps of any kind took over 2 ms. */
+    float step_max_ms[3];
+    uint64_t slow_steps;
 } fb_server_stats;
 fb_status fb_server_get_stats(fb_server *s, fb_server_stats *out);
 /* Ids of the LUT sets currently resident (finished and not evicted). */

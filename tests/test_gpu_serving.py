@@ -26,7 +26,12 @@ def fetch(server, lut, p):
     return {k: server.download(lut, k, s) for k, s in shapes(p).items()}
 
 
-def test_alone_vs_mix(builder):
+@pytest.mark.parametrize("share", [0, 60])
+def test_alone_vs_mix(builder, share):
+    """Alone: full-occupancy launches.  In the mix: the same (share 0) or
+    persistent launches holding 60% of the resident-block slots (share 60);
+    either way every texel is computed by the same code on the same inputs,
+    so the results are bit-identical."""
     target = Parameters().replace(**DUMP_SIZES, order=4, flags=abi.FB_FLAG_PRECISE)
     # alone
     s = Server(builder, 8 * GB, keep_outputs=True)
@@ -41,7 +46,7 @@ def test_alone_vs_mix(builder):
     s.close()
     # in a mix: other precomputes (batched with the target) and renders
     rng = np.random.default_rng(5)
-    s = Server(builder, 8 * GB, keep_outputs=True)
+    s = Server(builder, 8 * GB, keep_outputs=True, precompute_share_pct=share)
     others = [random_atmosphere(rng, True, resolution="half") for _ in range(3)]
     s.submit_precompute(others[0])
     _, lut = s.submit_precompute(target)
