@@ -218,6 +218,54 @@ __device__ __forceinline__ float3 get_transmittance(const DevParams& P,
     }
     return make_float3(fminf(n.x / q.x, 1.0f), fminf(n.y / q.y, 1.0f), fminf(n.z / q.z, 1.0f));
 }
+// Precision mode: transmittance lookups in extended range.  The bilinear
+// lookup of transmittance.h:26-33 interpolates T_k = exp(-tau_k) over four
+// texels; from the optical depths tau_k (the transmittance pass stores
+// them beside T) the same sum is exp(-tau_0) * sum_k w_k exp(tau_0 - tau_k),
+// tau_0 the smallest depth among taps of nonzero weight, evaluated in FP64.
+// Long horizontal paths (tau > 87) have T below binary32's range; the
+// quotient of two such lookups in get_transmittance then loses every digit,
+// while this form keeps them.  Returns log(T) per channel.
+__device__ __forceinline__ void log_transmittance_to_top(const DevParams& P,
+                                                         const float4* __restrict__ OD,
+                                                         double r, double mu, double out[3]) {
+    double u, v;
+    transmittance_uv<double>(P, r, mu, u, v);
+    const Taps tx = linear_taps<double>(u, P.t_mu), ty = linear_taps<double>(v, P.t_r);
+    const float4 q[4] = {OD[ty.i0 * P.t_mu + tx.i0], OD[ty.i0 * P.t_mu + tx.i1],
+                         OD[ty.i1 * P.t_mu + tx.i0], OD[ty.i1 * P.t_mu + tx.i1]};
+    const double wx = tx.w, wy = ty.w;
+    const double w[4] = {(1.0 - wx) * (1.0 - wy), wx * (1.0 - wy), (1.0 - wx) * wy, wx * wy};
+    for (int c = 0; c < 3; ++c) {
+        double t[4];
+        for (int k = 0; k < 4; ++k) t[k] = c == 0 ? q[k].x : (c == 1 ? q[k].y : q[k].z);
+        double t0 = 1e300;
+        for (int k = 0; k < 4; ++k)
+            if (w[k] > 0.0) t0 = fmin(t0, t[k]);
+        double s = 0.0;
+        for (int k = 0; k < 4; ++k)
+            if (w[k] > 0.0) s += w[k] * exp(t0 - t[k]);
+        out[c] = log(s) - t0;
+    }
+}
+// get_transmittance (transmittance.h:35-61) in extended range.
+__device__ __forceinline__ float3 get_transmittance_ext(const DevParams& P,
+                                                       const float4* __restrict__ OD, double r,
+                                                       double mu, double d, bool ground) {
+    const double r_d = clamp_radius<double>(P, sqrt(d * d + 2.0 * r * mu * d + r * r));
+    const double mu_d = clamp_cosine<double>((r * mu + d) / r_d);
+    double n[3], q[3];
+    if (ground) {
+        log_transmittance_to_top(P, OD, r_d, -mu_d, n);
+        log_transmittance_to_top(P, OD, r, -mu, q);
+    } else {
+        log_transmittance_to_top(P, OD, r, mu, n);
+        log_transmittance_to_top(P, OD, r_d, mu_d, q);
+    }
+    return make_float3(float(fmin(exp(n[0] - q[0]), 1.0)), float(fmin(exp(n[1] - q[1]), 1.0)),
+                       float(fmin(exp(n[2] - q[2]), 1.0)));
+}
+
 // transmittance.h:63-74
 template <typename G>
 __device__ __forceinline__ float3 transmittance_to_sun(const DevParams& P,

@@ -193,8 +193,10 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
         if (st != FB_OK) break;
         plan->members.push_back(pa);
         dps[i] = a->dev;
-        luts[i] = fbk::LutPtrs{a->T.p,  a->E.p,   a->S.p, pa->dE.p, pa->dR.p,
-                               pa->dM.p, pa->dMS.p, pa->J.p, pa->MR.p};
+        luts[i] = fbk::LutPtrs{a->T.p,   a->E.p,   a->S.p, pa->dE.p,
+                               pa->dR.p,  pa->dM.p,  pa->dMS.p,
+                               pa->J.p,   pa->MR.p,
+                               (ps[i].flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr};
     }
     bool uploaded = false;  // an asynchronous copy into plan->block is enqueued on s
     auto fail = [&](fb_status code) {

@@ -34,6 +34,7 @@ hipError_t upload_tables();
 // array of these, indexed by the tile's atmosphere slot).
 struct LutPtrs {
     float4 *T, *E, *S, *dE, *dR, *dM, *dMS, *J, *MR;
+    float4* OD;  // optical depths beside T (precision mode; else null)
 };
 
 // Passes of a batched schedule (values match fb_pass).
@@ -88,14 +89,15 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* d_pa
 // Blocks of a batched launch of `pass` resident on the device at once.
 unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threads, size_t lds);
 
-hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s);
+// OD: optical-depth LUT written beside T (precision mode), or null.
+hipError_t launch_transmittance(const DevParams& P, float4* T, float4* OD, hipStream_t s);
 hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,
                                     hipStream_t s);
-hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
-                                    float4* dM, float4* S, float4* MR, const Tuning& tu,
-                                    hipStream_t s);
+hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* OD,
+                                    float4* dR, float4* dM, float4* S, float4* MR,
+                                    const Tuning& tu, hipStream_t s);
 hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
-                                     const float4* dR, const float4* dM, const float4* dMS,
+                                     const float4* OD, const float4* dR, const float4* dM, const float4* dMS,
                                      const float4* dE, float4* J, const Tuning& tu,
                                      hipStream_t s);
 // `order` here is the number of bounces summed (the loop's n - 1).

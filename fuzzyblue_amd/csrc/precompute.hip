@@ -116,7 +116,7 @@ struct Sum {
 // matches the reference's; the precision mode splits the sum over 8 lanes
 // with compensated partial sums.
 template <typename G, bool COMP, int LANES>
-__device__ __forceinline__ void transmittance_block(const DevParams& P, const int bx, const int by, const int bz, float4* __restrict__ T) {
+__device__ __forceinline__ void transmittance_block(const DevParams& P, const int bx, const int by, const int bz, float4* __restrict__ T, float4* __restrict__ OD) {
     const int sub = threadIdx.x % LANES;
     const int texel = (bx * blockDim.x + threadIdx.x) / LANES;
     const int n = P.t_mu * P.t_r;
@@ -146,15 +146,16 @@ __device__ __forceinline__ void transmittance_block(const DevParams& P, const in
         LO += __shfl_xor(LO, off, 64);
     }
     if (!active || sub != 0) return;
-    float4 out;
-    out.x = expf(-(P.rayleigh_scattering[0] * LR + P.mie_extinction[0] * LM +
-                   P.absorption_extinction[0] * LO));
-    out.y = expf(-(P.rayleigh_scattering[1] * LR + P.mie_extinction[1] * LM +
-                   P.absorption_extinction[1] * LO));
-    out.z = expf(-(P.rayleigh_scattering[2] * LR + P.mie_extinction[2] * LM +
-                   P.absorption_extinction[2] * LO));
-    out.w = 1.0f;
-    T[j * P.t_mu + i] = out;
+    const float t0 = P.rayleigh_scattering[0] * LR + P.mie_extinction[0] * LM +
+                     P.absorption_extinction[0] * LO;
+    const float t1 = P.rayleigh_scattering[1] * LR + P.mie_extinction[1] * LM +
+                     P.absorption_extinction[1] * LO;
+    const float t2 = P.rayleigh_scattering[2] * LR + P.mie_extinction[2] * LM +
+                     P.absorption_extinction[2] * LO;
+    T[j * P.t_mu + i] = make_float4(expf(-t0), expf(-t1), expf(-t2), 1.0f);
+    // Precision mode: the optical depth itself, for transmittances below
+    // binary32's range (see log_transmittance_to_top).
+    if (OD) OD[j * P.t_mu + i] = make_float4(t0, t1, t2, 0.0f);
 }
 
 // ====================================================== 2. direct irradiance
@@ -193,6 +194,7 @@ __device__ __forceinline__ void direct_irradiance_block(const DevParams& P, cons
 template <typename G, bool COMP>
 __device__ __forceinline__ void single_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                            const float4* __restrict__ T,
+                                                           const float4* __restrict__ OD,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
                                                            float4* __restrict__ S,
@@ -216,7 +218,11 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
         const int i = tid;
         G d_i = G(i) * dx;
         G r_d = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r));
-        float3 tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
+        float3 tv;
+        if constexpr (kPrecise)
+            tv = get_transmittance_ext(P, OD, r, mu, d_i, ground);
+        else
+            tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
         float w = (i == 0 || i == kMsSamples - 1) ? 0.5f : 1.0f;
         float alt = float(r_d - G(P.bottom));
         G sin_h = G(P.bottom) / r_d;
@@ -488,7 +494,7 @@ inline size_t density_lds_bytes(int nu, int i_mus, bool single) {
 // (phi_m, phi_{31-m}): both share cos(phi), hence nu2 and the two phase
 // functions; their sin(phi) terms have opposite signs.
 template <typename G, bool SINGLE>
-__device__ __forceinline__ void scattering_density_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ T, const float4* __restrict__ dR,
+__device__ __forceinline__ void scattering_density_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ T, const float4* __restrict__ OD, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J) {
     extern __shared__ float4 s_dyn[];
@@ -529,7 +535,11 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         float ginv = 0.0f;
         if (g_l) {
             G dg = distance_to_bottom<G>(P, r, ct);
-            float3 tg = get_transmittance<G>(P, T, r, ct, dg, true);
+            float3 tg;
+            if constexpr (sizeof(G) == sizeof(double))
+                tg = get_transmittance_ext(P, OD, r, ct, dg, true);
+            else
+                tg = get_transmittance<G>(P, T, r, ct, dg, true);
             gv = make_float4(tg.x * P.albedo[0] * (1.0f / kPi), tg.y * P.albedo[1] * (1.0f / kPi),
                              tg.z * P.albedo[2] * (1.0f / kPi), float(dg));
             float4 o = c_den_dir[l * kDenPhi];
@@ -942,8 +952,9 @@ __device__ __forceinline__ void decode_tile(uint32_t t, int& atm, int& local) {
 }
 
 template <typename G, bool COMP, int LANES>
-__global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T) {
-    transmittance_block<G, COMP, LANES>(P, blockIdx.x, 0, 0, T);
+__global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T,
+                                                       float4* __restrict__ OD) {
+    transmittance_block<G, COMP, LANES>(P, blockIdx.x, 0, 0, T, OD);
 }
 template <typename G, bool COMP, int LANES>
 __global__ __launch_bounds__(256) void kb_transmittance(const DevParams* __restrict__ Ps,
@@ -956,7 +967,7 @@ __global__ __launch_bounds__(256) void kb_transmittance(const DevParams* __restr
         if (t != blockIdx.x) __syncthreads();
         int atm, local;
         decode_tile(tiles[t], atm, local);
-        transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, L[atm].T);
+        transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, L[atm].T, L[atm].OD);
     }
 }
 
@@ -999,13 +1010,14 @@ __device__ __forceinline__ unsigned logical_block(bool remap) {
 
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const float4* __restrict__ T,
+                                                           const float4* __restrict__ OD,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
                                                            float4* __restrict__ S,
                                                            float4* __restrict__ MR,
                                                            unsigned remap) {
     const unsigned b = logical_block(remap);
-    single_scattering_block<G, COMP>(P, b % P.s_mu, b / P.s_mu, 0, T, dR, dM, S, MR);
+    single_scattering_block<G, COMP>(P, b % P.s_mu, b / P.s_mu, 0, T, OD, dR, dM, S, MR);
 }
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
@@ -1020,19 +1032,19 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs& l = L[atm];
-        single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.dR, l.dM, l.S,
-                                         l.MR);
+        single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.OD, l.dR,
+                                         l.dM, l.S, l.MR);
     }
 }
 
 template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256, 8) void k_scattering_density(
-    DevParams P, const float4* __restrict__ T, const float4* __restrict__ dR,
-    const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
-    float4* __restrict__ J, unsigned remap) {
+    DevParams P, const float4* __restrict__ T, const float4* __restrict__ OD,
+    const float4* __restrict__ dR, const float4* __restrict__ dM, const float4* __restrict__ dMS,
+    const float4* __restrict__ dE, float4* __restrict__ J, unsigned remap) {
     const unsigned b = logical_block(remap), gx = gridDim.x;
-    scattering_density_block<G, SINGLE>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus), T, dR,
-                                        dM, dMS, dE, J);
+    scattering_density_block<G, SINGLE>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus), T, OD,
+                                        dR, dM, dMS, dE, J);
 }
 template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams* __restrict__ Ps,
@@ -1049,7 +1061,8 @@ __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams*
         const LutPtrs& l = L[atm];
         const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
         scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
-                                            local / (gx * P.s_mus), l.T, l.dR, l.dM, l.dMS, l.dE, l.J);
+                                            local / (gx * P.s_mus), l.T, l.OD, l.dR, l.dM, l.dMS,
+                                            l.dE, l.J);
     }
 }
 
@@ -1128,12 +1141,12 @@ __global__ void kb_clear_irradiance(const DevParams* __restrict__ Ps,
 // ========================================================== launch wrappers
 static inline unsigned cdiv(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
 
-hipError_t launch_transmittance(const DevParams& P, float4* T, hipStream_t s) {
+hipError_t launch_transmittance(const DevParams& P, float4* T, float4* OD, hipStream_t s) {
     size_t texels = size_t(P.t_mu) * P.t_r;
     if (P.flags & 1)
-        k_transmittance<double, true, 8><<<cdiv(texels * 8, 256), 256, 0, s>>>(P, T);
+        k_transmittance<double, true, 8><<<cdiv(texels * 8, 256), 256, 0, s>>>(P, T, OD);
     else
-        k_transmittance<float, false, 1><<<cdiv(texels, 64), 64, 0, s>>>(P, T);
+        k_transmittance<float, false, 1><<<cdiv(texels, 64), 64, 0, s>>>(P, T, nullptr);
     return hipGetLastError();
 }
 
@@ -1147,21 +1160,22 @@ hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4*
     return hipGetLastError();
 }
 
-hipError_t launch_single_scattering(const DevParams& P, const float4* T, float4* dR,
-                                    float4* dM, float4* S, float4* MR, const Tuning& tu,
-                                    hipStream_t s) {
+hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* OD,
+                                    float4* dR, float4* dM, float4* S, float4* MR,
+                                    const Tuning& tu, hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassSingle, 2, tu);
     const unsigned remap = (tu.remap >> kPassSingle) & 1u;
     dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
-        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR, remap);
+        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, OD, dR, dM, S, MR, remap);
     else
-        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, dR, dM, S, MR, remap);
+        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, OD, dR, dM, S, MR, remap);
     return hipGetLastError();
 }
 
 hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
-                                     const float4* dR, const float4* dM, const float4* dMS,
+                                     const float4* OD, const float4* dR, const float4* dM,
+                                     const float4* dMS,
                                      const float4* dE, float4* J, const Tuning& tu,
                                      hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassDensity, order, tu);
@@ -1173,13 +1187,17 @@ hipError_t launch_scattering_density(const DevParams& P, int order, const float4
     size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
     const bool precise = P.flags & 1;
     if (precise && single)
-        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
+                                                                     remap);
     else if (precise)
-        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
+                                                                     remap);
     else if (single)
-        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
+                                                                     remap);
     else
-        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, T, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
+                                                                     remap);
     return hipGetLastError();
 }
 

@@ -165,6 +165,7 @@ void free_pending_scratch(fb_pending_atmosphere* pa) {
     free_buf(b, &pa->dMS);
     free_buf(b, &pa->J);
     free_buf(b, &pa->MR);
+    free_buf(b, &pa->OD);
 }
 
 void free_atmosphere_buffers(fb_atmosphere* a) {
@@ -196,6 +197,7 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
     if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dMS);
     if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->J);
     if (st == FB_OK) st = alloc_buf(b, s.nmr, &pa->MR);
+    if (st == FB_OK) st = alloc_buf(b, s.nt, &pa->OD);
     hipError_t e = hipSuccess;
     if (st == FB_OK) e = hipEventCreateWithFlags(&pa->done, hipEventDisableTiming);
     if (st == FB_OK && e == hipSuccess) e = hipEventCreateWithFlags(&a->ready, hipEventDisableTiming);
@@ -251,17 +253,20 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
     Sizes sz = sizes_of(a->params);
     const fbk::Tuning tu = builder_tuning(a->builder, a->params);
     FB_MARK(0, 0);
-    FB_HIP_TRY(fbk::launch_transmittance(P, a->T.p, s));
+    // Precision mode keeps the optical depths beside T (extended-range
+    // segment transmittance); FP32 mode follows the reference's quotient.
+    float4* OD = (a->params.flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr;
+    FB_HIP_TRY(fbk::launch_transmittance(P, a->T.p, OD, s));
     FB_MARK(FB_PASS_TRANSMITTANCE, 1);
     FB_HIP_TRY(fbk::launch_direct_irradiance(P, a->T.p, pa->dE.p, s));
     FB_MARK(FB_PASS_DIRECT_IRRADIANCE, 1);
     FB_HIP_TRY(
-        fbk::launch_single_scattering(P, a->T.p, pa->dR.p, pa->dM.p, a->S.p, pa->MR.p, tu, s));
+        fbk::launch_single_scattering(P, a->T.p, OD, pa->dR.p, pa->dM.p, a->S.p, pa->MR.p, tu, s));
     FB_MARK(FB_PASS_SINGLE_SCATTERING, 1);
     FB_HIP_TRY(fbk::launch_fill_zero(a->E.p, sz.ni, s));
     FB_MARK(FB_PASS_CLEAR_IRRADIANCE, 1);
     for (int order = 2; order <= int(a->params.order); ++order) {
-        FB_HIP_TRY(fbk::launch_scattering_density(P, order, a->T.p, pa->dR.p, pa->dM.p,
+        FB_HIP_TRY(fbk::launch_scattering_density(P, order, a->T.p, OD, pa->dR.p, pa->dM.p,
                                                   pa->dMS.p, pa->dE.p, pa->J.p, tu, s));
         FB_MARK(FB_PASS_SCATTERING_DENSITY, order);
         FB_HIP_TRY(fbk::launch_indirect_irradiance(P, order - 1, pa->dR.p, pa->dM.p, pa->dMS.p,
@@ -450,7 +455,7 @@ size_t fb_parameters_footprint(const fb_parameters* p, size_t* resident, size_t*
     if (!p) return 0;
     Sizes s = sizes_of(*p);
     size_t res = (s.nt + s.ni + s.ns) * sizeof(float4);
-    size_t scr = (s.ni + 4 * s.ns + s.nmr) * sizeof(float4);
+    size_t scr = (s.nt + s.ni + 4 * s.ns + s.nmr) * sizeof(float4);
     if (resident) *resident = res;
     if (scratch) *scratch = scr;
     return res + scr;

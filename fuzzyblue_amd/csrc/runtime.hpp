@@ -106,7 +106,7 @@ struct fb_atmosphere {
 struct fb_pending_atmosphere {
     fb_builder* builder = nullptr;
     fb_atmosphere* atm = nullptr;
-    fbrt::Buf dE, dR, dM, dMS, J, MR;
+    fbrt::Buf dE, dR, dM, dMS, J, MR, OD;
     hipEvent_t done = nullptr;
     hipGraphExec_t graph = nullptr;
     std::shared_ptr<void> batch;  // shared state of a batched build

@@ -147,22 +147,23 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
     float4 *T = a->T.p, *E = a->E.p, *S = a->S.p;
     float4 *dE = pa->dE.p, *dR = pa->dR.p, *dM = pa->dM.p, *dMS = pa->dMS.p, *J = pa->J.p,
            *MR = pa->MR.p;
+    const float4* OD = (p->flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr;
     // (The repeated launches rewrite the scratch set's LUTs, and the
     // indirect and multiple passes accumulate into E and S; the set is
     // discarded afterwards.)
     st = tune(0, fbk::kPassSingle, 1, &fbk::Tuning::single, 1, fbk::kPassSingle,
               [&](const fbk::Tuning& t) {
-        return fbk::launch_single_scattering(P, T, dR, dM, S, MR, t, s);
+        return fbk::launch_single_scattering(P, T, OD, dR, dM, S, MR, t, s);
     });
     if (st == FB_OK)
         st = tune(1, fbk::kPassDensity, 2, &fbk::Tuning::density2, 1, -1,
                   [&](const fbk::Tuning& t) {
-            return fbk::launch_scattering_density(P, 2, T, dR, dM, dMS, dE, J, t, s);
+            return fbk::launch_scattering_density(P, 2, T, OD, dR, dM, dMS, dE, J, t, s);
         });
     if (st == FB_OK)
         st = tune(2, fbk::kPassDensity, 3, &fbk::Tuning::density, 1, fbk::kPassDensity,
                   [&](const fbk::Tuning& t) {
-            return fbk::launch_scattering_density(P, 3, T, dR, dM, dMS, dE, J, t, s);
+            return fbk::launch_scattering_density(P, 3, T, OD, dR, dM, dMS, dE, J, t, s);
         });
     if (st == FB_OK)
         st = tune(3, fbk::kPassIndirect, 2, &fbk::Tuning::indirect1, 64, -1,

@@ -53,9 +53,10 @@ def test_footprint():
     tot = load().fb_parameters_footprint(C.byref(c), C.byref(res), C.byref(scr))
     ns = 256 * 128 * 32
     assert res.value == (256 * 64 + 64 * 16 + ns) * 16
-    # irradiance delta, four scattering-sized scratch LUTs and the
-    # multiple-scattering row table (3 float4 x 51 samples per (r, mu) row)
-    assert scr.value == (64 * 16 + 4 * ns + 32 * 128 * 3 * 51) * 16
+    # optical depths (transmittance-sized), irradiance delta, four
+    # scattering-sized scratch LUTs and the multiple-scattering row table
+    # (3 float4 x 51 samples per (r, mu) row)
+    assert scr.value == (256 * 64 + 64 * 16 + 4 * ns + 32 * 128 * 3 * 51) * 16
     assert tot == res.value + scr.value
 
 
