@@ -196,7 +196,8 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
         luts[i] = fbk::LutPtrs{a->T.p,   a->E.p,   a->S.p, pa->dE.p,
                                pa->dR.p,  pa->dM.p,  pa->dMS.p,
                                pa->J.p,   pa->MR.p,
-                               (ps[i].flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr};
+                               (ps[i].flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr,
+                               pa->DG.p};
     }
     bool uploaded = false;  // an asynchronous copy into plan->block is enqueued on s
     auto fail = [&](fb_status code) {
@@ -212,8 +213,8 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
     for (const auto& p : ps) max_order = std::max(max_order, int(p.order));
     std::vector<fbk::Tuning> tus(n);
     for (size_t i = 0; i < n; ++i) tus[i] = builder_tuning(b, ps[i]);
-    for (int pass : {fbk::kPassTransmittance, fbk::kPassDirect, fbk::kPassSingle,
-                     fbk::kPassClear})
+    for (int pass : {fbk::kPassTransmittance, fbk::kPassDensityGeometry, fbk::kPassDirect,
+                     fbk::kPassSingle, fbk::kPassClear})
         if ((st = add_launch(*plan, ps, dps, tus, order_idx, pass, 1)) != FB_OK) return fail(st);
     for (int order = 2; order <= max_order; ++order)
         for (int pass : {fbk::kPassDensity, fbk::kPassIndirect, fbk::kPassMultiple})

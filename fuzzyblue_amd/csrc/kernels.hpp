@@ -35,6 +35,7 @@ hipError_t upload_tables();
 struct LutPtrs {
     float4 *T, *E, *S, *dE, *dR, *dM, *dMS, *J, *MR;
     float4* OD;  // optical depths beside T (precision mode; else null)
+    float4* DG;  // density geometry table (dg_* below)
 };
 
 // Passes of a batched schedule (values match fb_pass).
@@ -45,8 +46,22 @@ enum BatchPass {
     kPassClear = 3,
     kPassDensity = 4,
     kPassIndirect = 5,
-    kPassMultiple = 6
+    kPassMultiple = 6,
+    kPassDensityGeometry = 7
 };
+
+// Density geometry table, built once per LUT set after the transmittance
+// pass (it depends on r, theta and mu_s only, never on the order): for each
+// r index z, 16 ground entries (T_g * albedo / pi rgb, d_g) and 16 row
+// entries (mu taps i0, i1 as int bits, weight, 1/|ground normal| or 0 when
+// the row misses the ground), then the r taps; after all z, for each mu_s
+// index j the x taps of nu slots -1..NU.
+constexpr int kDgRow = 2 * 16 + 1;
+__host__ __device__ inline size_t dg_row(int z) { return size_t(z) * kDgRow; }
+__host__ __device__ inline size_t dg_x(int R, int j, int NU) {
+    return size_t(R) * kDgRow + size_t(j) * (NU + 2);
+}
+__host__ __device__ inline size_t dg_count(int R, int MUS, int NU) { return dg_x(R, MUS, NU); }
 
 // Block sizes an autotuner may choose per LUT-size class; 0 = default.
 // None of them changes the arithmetic of a texel (each texel's sums run in
@@ -93,11 +108,13 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
 hipError_t launch_transmittance(const DevParams& P, float4* T, float4* OD, hipStream_t s);
 hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,
                                     hipStream_t s);
+hipError_t launch_density_geometry(const DevParams& P, const float4* T, const float4* OD,
+                                   float4* DG, hipStream_t s);
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* OD,
                                     float4* dR, float4* dM, float4* S, float4* MR,
                                     const Tuning& tu, hipStream_t s);
-hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
-                                     const float4* OD, const float4* dR, const float4* dM, const float4* dMS,
+hipError_t launch_scattering_density(const DevParams& P, int order, const float4* DG,
+                                     const float4* dR, const float4* dM, const float4* dMS,
                                      const float4* dE, float4* J, const Tuning& tu,
                                      hipStream_t s);
 // `order` here is the number of bounces summed (the loop's n - 1).

@@ -490,11 +490,69 @@ inline size_t density_lds_bytes(int nu, int i_mus, bool single) {
            sizeof(ProfEntry) * (size_t(ntex) * kDenTheta * (nu + 1) + i_mus + 1);
 }
 
+// The density geometry table (dg_row / dg_x) of one LUT set; block b < R
+// fills the rows of r index b, block R + j the x taps of mu_s index j.
+// scattering_density.comp:34-66 for the directions, :45-60 for the ground
+// term; the taps are GetScatteringTextureUvwzFromRMuMuSNu's (scattering.h
+// :7-60) for r = r(z), mu = cos(theta_l), mu_s = mu_s(j).
+template <typename G>
+__device__ __forceinline__ void density_geometry_block(const DevParams& P, const int b,
+                                                       const float4* __restrict__ T,
+                                                       const float4* __restrict__ OD,
+                                                       float4* __restrict__ DG) {
+    const int tid = threadIdx.x;
+    const int R = P.s_r, NU = P.s_nu;
+    if (b < R) {
+        const int z = b;
+        const G r = scattering_texel_coord<G>(P, 0u, 0u, unsigned(z)).r;
+        float4* row = DG + dg_row(z);
+        if (tid < kDenTheta) {
+            const int l = tid;
+            const G ct = den_cos<G>(l);
+            const bool g_l = ray_intersects_ground<G>(P, r, ct);
+            const Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r, ct, g_l), P.s_mu);
+            float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float ginv = 0.0f;
+            if (g_l) {
+                const G dg = distance_to_bottom<G>(P, r, ct);
+                float3 tg;
+                if constexpr (sizeof(G) == sizeof(double))
+                    tg = get_transmittance_ext(P, OD, r, ct, dg, true);
+                else
+                    tg = get_transmittance<G>(P, T, r, ct, dg, true);
+                gv = make_float4(tg.x * P.albedo[0] * (1.0f / kPi),
+                                 tg.y * P.albedo[1] * (1.0f / kPi),
+                                 tg.z * P.albedo[2] * (1.0f / kPi), float(dg));
+                // The ground normal's length does not depend on phi.
+                const float4 o = c_den_dir[l * kDenPhi];
+                const G st2 = G(o.x) * G(o.x) + G(o.y) * G(o.y);
+                const G gz = r + ct * dg;
+                ginv = float(G(1) / sqrt(st2 * dg * dg + gz * gz));
+            }
+            row[l] = gv;
+            row[kDenTheta + l] =
+                make_float4(__int_as_float(ty.i0), __int_as_float(ty.i1), ty.w, ginv);
+        } else if (tid == kDenTheta) {
+            const Taps tz = linear_taps<G>(scattering_u_r<G>(P, r), P.s_r);
+            row[2 * kDenTheta] =
+                make_float4(__int_as_float(tz.i0), __int_as_float(tz.i1), tz.w, 0.0f);
+        }
+    } else {
+        const int j = b - R;
+        if (tid < NU + 2) {
+            const G mu_s = scattering_texel_coord<G>(P, unsigned(j), 0u, 0u).mu_s;
+            const Taps tx = slice_taps<G>(P, tid - 1, scattering_u_mu_s<G>(P, mu_s));
+            DG[dg_x(R, j, NU) + tid] =
+                make_float4(__int_as_float(tx.i0), __int_as_float(tx.i1), tx.w, 0.0f);
+        }
+    }
+}
+
 // Per-sample body of the density quadrature for one pair of directions
 // (phi_m, phi_{31-m}): both share cos(phi), hence nu2 and the two phase
 // functions; their sin(phi) terms have opposite signs.
 template <typename G, bool SINGLE>
-__device__ __forceinline__ void scattering_density_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ T, const float4* __restrict__ OD, const float4* __restrict__ dR,
+__device__ __forceinline__ void scattering_density_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ DG, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J) {
     extern __shared__ float4 s_dyn[];
@@ -517,47 +575,25 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     ProfEntry* s_prof = reinterpret_cast<ProfEntry*>(s_eraw + NE);  // [NTEX][16][nent]
     ProfEntry* s_erow = s_prof + NTEX * kDenTheta * nent;    // [NE+1]
 
-    // 0. Tile geometry, once per theta row / nu slot: r and mu_s are shared
-    // by every texel of the tile (texel x = j, y = 0).  Only the threads
-    // that need them evaluate the G-precision maps, and the resulting taps
-    // go through LDS, so no G value stays live across the gather loops.
+    // 0. Tile geometry from the density geometry table: the mu taps and
+    // ground terms of each theta row (they depend on r only), the r taps,
+    // and the x taps of each nu slot (they depend on mu_s only).
     const int slot0 = int(blockDim.x) >= 64 + nslots ? 64 : kDenTheta;
     if (tid < kDenTheta) {
-        // Ground terms per theta row (scattering_density.comp:45-60).  The
-        // ground normal's length does not depend on phi.
-        const TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
-        const G r = tc.r;
-        int l = tid;
-        G ct = den_cos<G>(l);
-        bool g_l = ray_intersects_ground<G>(P, r, ct);
-        s_ty[l] = linear_taps<G>(scattering_u_mu<G>(P, r, ct, g_l), MU);
-        float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float ginv = 0.0f;
-        if (g_l) {
-            G dg = distance_to_bottom<G>(P, r, ct);
-            float3 tg;
-            if constexpr (sizeof(G) == sizeof(double))
-                tg = get_transmittance_ext(P, OD, r, ct, dg, true);
-            else
-                tg = get_transmittance<G>(P, T, r, ct, dg, true);
-            gv = make_float4(tg.x * P.albedo[0] * (1.0f / kPi), tg.y * P.albedo[1] * (1.0f / kPi),
-                             tg.z * P.albedo[2] * (1.0f / kPi), float(dg));
-            float4 o = c_den_dir[l * kDenPhi];
-            G st2 = G(o.x) * G(o.x) + G(o.y) * G(o.y);
-            G gz = r + ct * dg;
-            ginv = float(G(1) / sqrt(st2 * dg * dg + gz * gz));
-        }
-        s_gv[l] = gv;
-        s_ginv[l] = ginv;
-        unsigned long long b = __ballot(g_l);
-        if (l == 0) s_ground_mask = int(b & 0xffffull);
+        const float4* dgz = DG + dg_row(z);
+        const float4 t = dgz[kDenTheta + tid];
+        s_gv[tid] = dgz[tid];
+        s_ginv[tid] = t.w;
+        s_ty[tid] = Taps{__float_as_int(t.x), __float_as_int(t.y), t.z};
+        const unsigned long long b = __ballot(t.w != 0.0f);
+        if (tid == 0) s_ground_mask = int(b & 0xffffull);
     } else if (tid >= slot0 && tid < slot0 + nslots) {
-        // x taps of nu slot k = -1..NU (a second wavefront when the block
-        // has one).
-        const TexelCoord<G> tc = scattering_texel_coord<G>(P, unsigned(j), 0u, unsigned(z));
-        const G u_mu_s = scattering_u_mu_s<G>(P, tc.mu_s);
-        s_tx[tid - slot0] = slice_taps<G>(P, tid - slot0 - 1, u_mu_s);
-        if (tid == slot0) s_tz = linear_taps<G>(scattering_u_r<G>(P, tc.r), P.s_r);
+        const float4 x = DG[dg_x(P.s_r, j, NU) + (tid - slot0)];
+        s_tx[tid - slot0] = Taps{__float_as_int(x.x), __float_as_int(x.y), x.z};
+        if (tid == slot0) {
+            const float4 c = DG[dg_row(z) + 2 * kDenTheta];
+            s_tz = Taps{__float_as_int(c.x), __float_as_int(c.y), c.z};
+        }
     }
     // delta_irradiance at r = bottom (irradiance.h:20-30 with x_r = 0).
     {
@@ -1039,12 +1075,30 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
 
 template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256, 8) void k_scattering_density(
-    DevParams P, const float4* __restrict__ T, const float4* __restrict__ OD,
-    const float4* __restrict__ dR, const float4* __restrict__ dM, const float4* __restrict__ dMS,
-    const float4* __restrict__ dE, float4* __restrict__ J, unsigned remap) {
+    DevParams P, const float4* __restrict__ DG, const float4* __restrict__ dR,
+    const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
+    float4* __restrict__ J, unsigned remap) {
     const unsigned b = logical_block(remap), gx = gridDim.x;
-    scattering_density_block<G, SINGLE>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus), T, OD,
+    scattering_density_block<G, SINGLE>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus), DG,
                                         dR, dM, dMS, dE, J);
+}
+
+template <typename G>
+__global__ __launch_bounds__(64) void k_density_geometry(DevParams P, const float4* __restrict__ T,
+                                                         const float4* __restrict__ OD,
+                                                         float4* __restrict__ DG) {
+    density_geometry_block<G>(P, blockIdx.x, T, OD, DG);
+}
+template <typename G>
+__global__ __launch_bounds__(64) void kb_density_geometry(const DevParams* __restrict__ Ps,
+                                                          const LutPtrs* __restrict__ L,
+                                                          const uint32_t* __restrict__ tiles,
+                                                          unsigned ntiles) {
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        density_geometry_block<G>(Ps[atm], local, L[atm].T, L[atm].OD, L[atm].DG);
+    }
 }
 template <typename G, bool SINGLE>
 __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams* __restrict__ Ps,
@@ -1061,8 +1115,8 @@ __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams*
         const LutPtrs& l = L[atm];
         const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
         scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
-                                            local / (gx * P.s_mus), l.T, l.OD, l.dR, l.dM, l.dMS,
-                                            l.dE, l.J);
+                                            local / (gx * P.s_mus), l.DG, l.dR, l.dM, l.dMS, l.dE,
+                                            l.J);
     }
 }
 
@@ -1160,6 +1214,16 @@ hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4*
     return hipGetLastError();
 }
 
+hipError_t launch_density_geometry(const DevParams& P, const float4* T, const float4* OD,
+                                   float4* DG, hipStream_t s) {
+    const unsigned blocks = unsigned(P.s_r + P.s_mus);
+    if (P.flags & 1)
+        k_density_geometry<double><<<blocks, 64, 0, s>>>(P, T, OD, DG);
+    else
+        k_density_geometry<float><<<blocks, 64, 0, s>>>(P, T, OD, DG);
+    return hipGetLastError();
+}
+
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* OD,
                                     float4* dR, float4* dM, float4* S, float4* MR,
                                     const Tuning& tu, hipStream_t s) {
@@ -1173,9 +1237,8 @@ hipError_t launch_single_scattering(const DevParams& P, const float4* T, const f
     return hipGetLastError();
 }
 
-hipError_t launch_scattering_density(const DevParams& P, int order, const float4* T,
-                                     const float4* OD, const float4* dR, const float4* dM,
-                                     const float4* dMS,
+hipError_t launch_scattering_density(const DevParams& P, int order, const float4* DG,
+                                     const float4* dR, const float4* dM, const float4* dMS,
                                      const float4* dE, float4* J, const Tuning& tu,
                                      hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassDensity, order, tu);
@@ -1187,17 +1250,13 @@ hipError_t launch_scattering_density(const DevParams& P, int order, const float4
     size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
     const bool precise = P.flags & 1;
     if (precise && single)
-        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
-                                                                     remap);
+        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     else if (precise)
-        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
-                                                                     remap);
+        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     else if (single)
-        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
-                                                                     remap);
+        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     else
-        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, T, OD, dR, dM, dMS, dE, J,
-                                                                     remap);
+        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     return hipGetLastError();
 }
 
@@ -1309,6 +1368,8 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
             return {unsigned(P.s_mu) * unsigned(P.s_r), threads, 0};
         case kPassClear:
             return {cdiv(ni, 256), 256, 0};
+        case kPassDensityGeometry:
+            return {unsigned(P.s_r + P.s_mus), 64, 0};
         case kPassDensity:
             return {cdiv(size_t(P.s_mu) * P.s_nu, threads) * unsigned(P.s_mus) * unsigned(P.s_r),
                     threads, density_lds_bytes(P.s_nu, P.i_mus, (order - 1) == 1)};
@@ -1351,6 +1412,12 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
             break;
         case kPassClear:
             kb_clear_irradiance<<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+            break;
+        case kPassDensityGeometry:
+            if (precise)
+                kb_density_geometry<double><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+            else
+                kb_density_geometry<float><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassDensity:
             if (precise && single)
@@ -1404,6 +1471,10 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
             break;
         case kPassClear:
             fn = (const void*)kb_clear_irradiance;
+            break;
+        case kPassDensityGeometry:
+            fn = precise ? (const void*)kb_density_geometry<double>
+                         : (const void*)kb_density_geometry<float>;
             break;
         case kPassDensity:
             fn = precise ? (single ? (const void*)kb_scattering_density<double, true>

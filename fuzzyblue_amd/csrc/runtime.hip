@@ -102,6 +102,8 @@ Sizes sizes_of(const fb_parameters& p) {
     s.ns = size_t(p.scattering_nu_size) * p.scattering_mu_s_size * p.scattering_mu_size *
            p.scattering_r_size;
     s.nmr = size_t(p.scattering_mu_size) * p.scattering_r_size * fbk::kMsRowFloat4s;
+    s.ndg = fbk::dg_count(int(p.scattering_r_size), int(p.scattering_mu_s_size),
+                          int(p.scattering_nu_size));
     return s;
 }
 
@@ -166,6 +168,7 @@ void free_pending_scratch(fb_pending_atmosphere* pa) {
     free_buf(b, &pa->J);
     free_buf(b, &pa->MR);
     free_buf(b, &pa->OD);
+    free_buf(b, &pa->DG);
 }
 
 void free_atmosphere_buffers(fb_atmosphere* a) {
@@ -198,6 +201,7 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
     if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->J);
     if (st == FB_OK) st = alloc_buf(b, s.nmr, &pa->MR);
     if (st == FB_OK) st = alloc_buf(b, s.nt, &pa->OD);
+    if (st == FB_OK) st = alloc_buf(b, s.ndg, &pa->DG);
     hipError_t e = hipSuccess;
     if (st == FB_OK) e = hipEventCreateWithFlags(&pa->done, hipEventDisableTiming);
     if (st == FB_OK && e == hipSuccess) e = hipEventCreateWithFlags(&a->ready, hipEventDisableTiming);
@@ -258,6 +262,8 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
     float4* OD = (a->params.flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr;
     FB_HIP_TRY(fbk::launch_transmittance(P, a->T.p, OD, s));
     FB_MARK(FB_PASS_TRANSMITTANCE, 1);
+    FB_HIP_TRY(fbk::launch_density_geometry(P, a->T.p, OD, pa->DG.p, s));
+    FB_MARK(FB_PASS_DENSITY_GEOMETRY, 1);
     FB_HIP_TRY(fbk::launch_direct_irradiance(P, a->T.p, pa->dE.p, s));
     FB_MARK(FB_PASS_DIRECT_IRRADIANCE, 1);
     FB_HIP_TRY(
@@ -266,7 +272,7 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
     FB_HIP_TRY(fbk::launch_fill_zero(a->E.p, sz.ni, s));
     FB_MARK(FB_PASS_CLEAR_IRRADIANCE, 1);
     for (int order = 2; order <= int(a->params.order); ++order) {
-        FB_HIP_TRY(fbk::launch_scattering_density(P, order, a->T.p, OD, pa->dR.p, pa->dM.p,
+        FB_HIP_TRY(fbk::launch_scattering_density(P, order, pa->DG.p, pa->dR.p, pa->dM.p,
                                                   pa->dMS.p, pa->dE.p, pa->J.p, tu, s));
         FB_MARK(FB_PASS_SCATTERING_DENSITY, order);
         FB_HIP_TRY(fbk::launch_indirect_irradiance(P, order - 1, pa->dR.p, pa->dM.p, pa->dMS.p,
@@ -455,7 +461,7 @@ size_t fb_parameters_footprint(const fb_parameters* p, size_t* resident, size_t*
     if (!p) return 0;
     Sizes s = sizes_of(*p);
     size_t res = (s.nt + s.ni + s.ns) * sizeof(float4);
-    size_t scr = (s.nt + s.ni + 4 * s.ns + s.nmr) * sizeof(float4);
+    size_t scr = (s.nt + s.ni + 4 * s.ns + s.nmr + s.ndg) * sizeof(float4);
     if (resident) *resident = res;
     if (scratch) *scratch = scr;
     return res + scr;

@@ -58,6 +58,7 @@ fbk::DevParams make_dev_params(const fb_parameters& p);
 struct Sizes {
     size_t nt, ni, ns;  // texel counts: transmittance, irradiance, scattering
     size_t nmr;         // float4s of the multiple-scattering row table
+    size_t ndg;         // float4s of the density geometry table
 };
 Sizes sizes_of(const fb_parameters& p);
 
@@ -106,7 +107,7 @@ struct fb_atmosphere {
 struct fb_pending_atmosphere {
     fb_builder* builder = nullptr;
     fb_atmosphere* atm = nullptr;
-    fbrt::Buf dE, dR, dM, dMS, J, MR, OD;
+    fbrt::Buf dE, dR, dM, dMS, J, MR, OD, DG;
     hipEvent_t done = nullptr;
     hipGraphExec_t graph = nullptr;
     std::shared_ptr<void> batch;  // shared state of a batched build

@@ -148,6 +148,7 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
     float4 *dE = pa->dE.p, *dR = pa->dR.p, *dM = pa->dM.p, *dMS = pa->dMS.p, *J = pa->J.p,
            *MR = pa->MR.p;
     const float4* OD = (p->flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr;
+    const float4* DG = pa->DG.p;
     // (The repeated launches rewrite the scratch set's LUTs, and the
     // indirect and multiple passes accumulate into E and S; the set is
     // discarded afterwards.)
@@ -158,12 +159,12 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
     if (st == FB_OK)
         st = tune(1, fbk::kPassDensity, 2, &fbk::Tuning::density2, 1, -1,
                   [&](const fbk::Tuning& t) {
-            return fbk::launch_scattering_density(P, 2, T, OD, dR, dM, dMS, dE, J, t, s);
+            return fbk::launch_scattering_density(P, 2, DG, dR, dM, dMS, dE, J, t, s);
         });
     if (st == FB_OK)
         st = tune(2, fbk::kPassDensity, 3, &fbk::Tuning::density, 1, fbk::kPassDensity,
                   [&](const fbk::Tuning& t) {
-            return fbk::launch_scattering_density(P, 3, T, OD, dR, dM, dMS, dE, J, t, s);
+            return fbk::launch_scattering_density(P, 3, DG, dR, dM, dMS, dE, J, t, s);
         });
     if (st == FB_OK)
         st = tune(3, fbk::kPassIndirect, 2, &fbk::Tuning::indirect1, 64, -1,

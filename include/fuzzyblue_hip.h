@@ -207,7 +207,12 @@ typedef enum fb_pass {
     FB_PASS_CLEAR_IRRADIANCE = 3,    /* src/precompute.rs:1802-1816 */
     FB_PASS_SCATTERING_DENSITY = 4,  /* shaders/scattering_density.comp */
     FB_PASS_INDIRECT_IRRADIANCE = 5, /* shaders/indirect_irradiance.comp */
-    FB_PASS_MULTIPLE_SCATTERING = 6  /* shaders/multiple_scattering.comp */
+    FB_PASS_MULTIPLE_SCATTERING = 6, /* shaders/multiple_scattering.comp */
+    FB_PASS_DENSITY_GEOMETRY = 7     /* no reference counterpart: per-set
+                                        table of the density pass's taps and
+                                        ground terms (scattering_density.comp
+                                        :34-66), built once after the
+                                        transmittance pass */
 } fb_pass;
 
 typedef struct fb_pass_timing {

@@ -56,7 +56,8 @@ def test_footprint():
     # optical depths (transmittance-sized), irradiance delta, four
     # scattering-sized scratch LUTs and the multiple-scattering row table
     # (3 float4 x 51 samples per (r, mu) row)
-    assert scr.value == (256 * 64 + 64 * 16 + 4 * ns + 32 * 128 * 3 * 51) * 16
+    # and the density geometry table (33 float4 per r, NU + 2 per mu_s)
+    assert scr.value == (256 * 64 + 64 * 16 + 4 * ns + 32 * 128 * 3 * 51 + 32 * 33 + 32 * 10) * 16
     assert tot == res.value + scr.value
 
 
