@@ -103,7 +103,7 @@ FB_PASS_CLEAR_IRRADIANCE = 3
 FB_PASS_SCATTERING_DENSITY = 4
 FB_PASS_INDIRECT_IRRADIANCE = 5
 FB_PASS_MULTIPLE_SCATTERING = 6
-FB_PASS_DENSITY_GEOMETRY = 7
+FB_PASS_GEOMETRY = 7
 
 FB_PASS_NAMES = {
     0: "transmittance",
@@ -113,7 +113,7 @@ FB_PASS_NAMES = {
     4: "scattering_density",
     5: "indirect_irradiance",
     6: "multiple_scattering",
-    7: "density_geometry",
+    7: "geometry",
 }
 
 

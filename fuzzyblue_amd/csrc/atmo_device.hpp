@@ -266,6 +266,29 @@ __device__ __forceinline__ float3 get_transmittance_ext(const DevParams& P,
                        float(fmin(exp(n[2] - q[2]), 1.0)));
 }
 
+// Precision mode's get_transmittance: the binary32 LUT quotient while its
+// denominator is a normal number well above binary32's range limit (the
+// quotient then keeps full precision), else the extended-range form.
+__device__ __forceinline__ float3 get_transmittance_precise(const DevParams& P,
+                                                           const float4* __restrict__ T,
+                                                           const float4* __restrict__ OD,
+                                                           double r, double mu, double d,
+                                                           bool ground) {
+    const double r_d = clamp_radius<double>(P, sqrt(d * d + 2.0 * r * mu * d + r * r));
+    const double mu_d = clamp_cosine<double>((r * mu + d) / r_d);
+    float3 n, q;
+    if (ground) {
+        n = transmittance_to_top<double>(P, T, r_d, -mu_d);
+        q = transmittance_to_top<double>(P, T, r, -mu);
+    } else {
+        n = transmittance_to_top<double>(P, T, r, mu);
+        q = transmittance_to_top<double>(P, T, r_d, mu_d);
+    }
+    if (fminf(q.x, fminf(q.y, q.z)) > 1e-30f)
+        return make_float3(fminf(n.x / q.x, 1.0f), fminf(n.y / q.y, 1.0f), fminf(n.z / q.z, 1.0f));
+    return get_transmittance_ext(P, OD, r, mu, d, ground);
+}
+
 // transmittance.h:63-74
 template <typename G>
 __device__ __forceinline__ float3 transmittance_to_sun(const DevParams& P,

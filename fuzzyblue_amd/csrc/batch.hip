@@ -197,7 +197,7 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
                                pa->dR.p,  pa->dM.p,  pa->dMS.p,
                                pa->J.p,   pa->MR.p,
                                (ps[i].flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr,
-                               pa->DG.p};
+                               pa->DG.p,  pa->SR.p};
     }
     bool uploaded = false;  // an asynchronous copy into plan->block is enqueued on s
     auto fail = [&](fb_status code) {
@@ -213,7 +213,7 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
     for (const auto& p : ps) max_order = std::max(max_order, int(p.order));
     std::vector<fbk::Tuning> tus(n);
     for (size_t i = 0; i < n; ++i) tus[i] = builder_tuning(b, ps[i]);
-    for (int pass : {fbk::kPassTransmittance, fbk::kPassDensityGeometry, fbk::kPassDirect,
+    for (int pass : {fbk::kPassTransmittance, fbk::kPassGeometry, fbk::kPassDirect,
                      fbk::kPassSingle, fbk::kPassClear})
         if ((st = add_launch(*plan, ps, dps, tus, order_idx, pass, 1)) != FB_OK) return fail(st);
     for (int order = 2; order <= max_order; ++order)

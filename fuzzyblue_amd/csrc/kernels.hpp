@@ -27,6 +27,12 @@ constexpr int kMsSamples = 51;
 // transmittance x trapezoid weight (rgb) and d_i, then (1/r_i, mu weight, r
 // weight), then the offsets of the four (r, mu) density rows.
 constexpr int kMsRowFloat4s = 3 * kMsSamples;
+// Single-scattering row table: per (r, mu) row and sample, the view
+// transmittance x trapezoid weight (rgb) and d_i; (r_d, Rayleigh density,
+// Mie density, sin of the horizon angle); (cos of the horizon angle,
+// d_min, d_max - d_min, T-row weight); (1/r_d, 1/(d_max - d_min),
+// top^2 - r_d^2); (T-row offsets as int bits, dx).
+constexpr int kSsRowFloat4s = 5 * kMsSamples;
 
 hipError_t upload_tables();
 
@@ -36,6 +42,7 @@ struct LutPtrs {
     float4 *T, *E, *S, *dE, *dR, *dM, *dMS, *J, *MR;
     float4* OD;  // optical depths beside T (precision mode; else null)
     float4* DG;  // density geometry table (dg_* below)
+    float4* SR;  // single-scattering row table (kSsRowFloat4s per (r, mu) row)
 };
 
 // Passes of a batched schedule (values match fb_pass).
@@ -47,7 +54,7 @@ enum BatchPass {
     kPassDensity = 4,
     kPassIndirect = 5,
     kPassMultiple = 6,
-    kPassDensityGeometry = 7
+    kPassGeometry = 7
 };
 
 // Density geometry table, built once per LUT set after the transmittance
@@ -108,11 +115,12 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
 hipError_t launch_transmittance(const DevParams& P, float4* T, float4* OD, hipStream_t s);
 hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4* dE,
                                     hipStream_t s);
-hipError_t launch_density_geometry(const DevParams& P, const float4* T, const float4* OD,
-                                   float4* DG, hipStream_t s);
-hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* OD,
-                                    float4* dR, float4* dM, float4* S, float4* MR,
-                                    const Tuning& tu, hipStream_t s);
+// The per-set geometry tables (kPassGeometry): DG, SR and MR.
+hipError_t launch_geometry(const DevParams& P, const float4* T, const float4* OD, float4* DG,
+                           float4* SR, float4* MR, hipStream_t s);
+hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* SR,
+                                    float4* dR, float4* dM, float4* S, const Tuning& tu,
+                                    hipStream_t s);
 hipError_t launch_scattering_density(const DevParams& P, int order, const float4* DG,
                                      const float4* dR, const float4* dM, const float4* dMS,
                                      const float4* dE, float4* J, const Tuning& tu,

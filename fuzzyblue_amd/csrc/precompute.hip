@@ -186,75 +186,90 @@ __device__ __forceinline__ void direct_irradiance_block(const DevParams& P, cons
 // single_scattering.comp:10-101.
 //
 // As for multiple scattering, the texels of one (r, mu) row share the view
-// ray: for each of the 51 samples the block computes once d_i, r_d, the
-// view transmittance T(r, mu, d_i), the densities at r_d and the
-// transmittance-LUT row taps and horizon terms of r_d.  Each texel then
-// evaluates only its sun transmittance T(r_d, mu_s_d) (one bilinear
+// ray: for each of the 51 samples, d_i, r_d, the view transmittance
+// T(r, mu, d_i), the densities at r_d and the transmittance-LUT row taps
+// and horizon terms of r_d are computed once per row (row_geometry_block,
+// in the per-set geometry pass) and read from the row table.  Each texel
+// then evaluates only its sun transmittance T(r_d, mu_s_d) (one bilinear
 // lookup) and accumulates.
+
+// The single- and multiple-scattering row tables of row (y, z): thread i
+// computes sample i.  The multiple-scattering passes march the same 51
+// samples (multiple_scattering.comp:16-47, the same dx and d_i).
+template <typename G>
+__device__ __forceinline__ void row_geometry_block(const DevParams& P, const int y, const int z,
+                                                   const float4* __restrict__ T,
+                                                   const float4* __restrict__ OD,
+                                                   float4* __restrict__ SR,
+                                                   float4* __restrict__ MR) {
+    const int tid = threadIdx.x;
+    if (tid >= kMsSamples) return;
+    const int MU = P.s_mu, W = P.s_nu * P.s_mus;
+    const TexelCoord<G> rc = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z));
+    const G r = rc.r, mu = rc.mu;
+    const bool ground = rc.ground;
+    const G dx = distance_to_nearest<G>(P, r, mu, ground) / G(kMsSamples - 1);
+    const int i = tid;
+    const G d_i = G(i) * dx;
+    const G r_d = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r));
+    float3 tv;
+    if constexpr (sizeof(G) == sizeof(double))
+        tv = get_transmittance_precise(P, T, OD, r, mu, d_i, ground);
+    else
+        tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
+    const float w = (i == 0 || i == kMsSamples - 1) ? 0.5f : 1.0f;
+    const float alt = float(r_d - G(P.bottom));
+    const G sin_h = G(P.bottom) / r_d;
+    const G cos_h = -sqrt(fmax(G(1) - sin_h * sin_h, G(0)));
+    // transmittance.h:7-24 terms that depend on r_d only.
+    const G H = sqrt(G(P.top) * G(P.top) - G(P.bottom) * G(P.bottom));
+    const G rho = safe_sqrt<G>(r_d * r_d - G(P.bottom) * G(P.bottom));
+    const G d_min = G(P.top) - r_d, d_max = rho + H;
+    const Taps tr = linear_taps<G>(tex_from_unit<G>(rho / H, P.t_r), P.t_r);
+    float4* o = SR + (size_t(z) * MU + y) * kSsRowFloat4s;
+    o[i] = make_float4(tv.x * w, tv.y * w, tv.z * w, float(d_i));
+    o[kMsSamples + i] = make_float4(float(r_d), profile_density(P.prof[0], alt),
+                                    profile_density(P.prof[1], alt), float(sin_h));
+    o[2 * kMsSamples + i] = make_float4(float(cos_h), float(d_min), float(d_max - d_min), tr.w);
+    o[3 * kMsSamples + i] = make_float4(float(G(1) / r_d), float(G(1) / (d_max - d_min)),
+                                        float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))), 0.0f);
+    o[4 * kMsSamples + i] = make_float4(__int_as_float(tr.i0 * P.t_mu),
+                                        __int_as_float(tr.i1 * P.t_mu), float(dx), 0.0f);
+    // Multiple scattering: T(r, mu, d_i) x dx x weight, then the (mu, r)
+    // taps of the sample point (r_i, mu_i).
+    const G mu_i = clamp_cosine<G>((r * mu + d_i) / r_d);
+    const Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r_d, mu_i, ground), MU);
+    const Taps tz = linear_taps<G>(scattering_u_r<G>(P, r_d), P.s_r);
+    const float f = float(dx) * w;
+    float4* m = MR + (size_t(z) * MU + y) * kMsRowFloat4s;
+    m[i] = make_float4(tv.x * f, tv.y * f, tv.z * f, float(d_i));
+    m[kMsSamples + i] = make_float4(float(G(1) / r_d), ty.w, tz.w, 0.0f);
+    reinterpret_cast<int4*>(m + 2 * kMsSamples)[i] =
+        make_int4((tz.i0 * MU + ty.i0) * W, (tz.i0 * MU + ty.i1) * W, (tz.i1 * MU + ty.i0) * W,
+                  (tz.i1 * MU + ty.i1) * W);
+}
+
 template <typename G, bool COMP>
 __device__ __forceinline__ void single_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                            const float4* __restrict__ T,
-                                                           const float4* __restrict__ OD,
+                                                           const float4* __restrict__ SR,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
-                                                           float4* __restrict__ S,
-                                                           float4* __restrict__ MR) {
-    __shared__ float4 s_tv[kMsSamples];   // T(r,mu,d_i) * weight (rgb), d_i
-    __shared__ float4 s_geo[kMsSamples];  // r_d, rho_rayleigh(r_d), rho_mie(r_d), sin_theta_h
-    __shared__ float4 s_lut[kMsSamples];  // cos_theta_h, d_min, d_max - d_min, T-LUT row weight
-    __shared__ float4 s_inv[kMsSamples];  // 1/r_d, 1/(d_max - d_min), top^2 - r_d^2 (precise)
-    __shared__ int2 s_trow[kMsSamples];   // T-LUT rows of r_d
+                                                           float4* __restrict__ S) {
+    __shared__ float4 s_row[kSsRowFloat4s];
+    const float4* s_tv = s_row;                    // T(r,mu,d_i) * weight (rgb), d_i
+    const float4* s_geo = s_row + kMsSamples;      // r_d, rho_rayleigh, rho_mie, sin_theta_h
+    const float4* s_lut = s_row + 2 * kMsSamples;  // cos_theta_h, d_min, d_max - d_min, T-row weight
+    const float4* s_inv = s_row + 3 * kMsSamples;  // 1/r_d, 1/(d_max - d_min), top^2 - r_d^2
+    const float4* s_trw = s_row + 4 * kMsSamples;  // T-row offsets (int bits), dx
     constexpr bool kPrecise = sizeof(G) == sizeof(double);
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
     const int y = bx, z = by;
     const int tid = threadIdx.x, nt = blockDim.x;
-
-    TexelCoord<G> rc = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z));
-    const G r = rc.r, mu = rc.mu;
-    const bool ground = rc.ground;
-    const G dx = distance_to_nearest<G>(P, r, mu, ground) / G(kMsSamples - 1);
-    if (tid < kMsSamples) {
-        const int i = tid;
-        G d_i = G(i) * dx;
-        G r_d = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r));
-        float3 tv;
-        if constexpr (kPrecise)
-            tv = get_transmittance_ext(P, OD, r, mu, d_i, ground);
-        else
-            tv = get_transmittance<G>(P, T, r, mu, d_i, ground);
-        float w = (i == 0 || i == kMsSamples - 1) ? 0.5f : 1.0f;
-        float alt = float(r_d - G(P.bottom));
-        G sin_h = G(P.bottom) / r_d;
-        G cos_h = -sqrt(fmax(G(1) - sin_h * sin_h, G(0)));
-        // transmittance.h:7-24 terms that depend on r_d only.
-        G H = sqrt(G(P.top) * G(P.top) - G(P.bottom) * G(P.bottom));
-        G rho = safe_sqrt<G>(r_d * r_d - G(P.bottom) * G(P.bottom));
-        G d_min = G(P.top) - r_d, d_max = rho + H;
-        Taps tr = linear_taps<G>(tex_from_unit<G>(rho / H, P.t_r), P.t_r);
-        s_tv[i] = make_float4(tv.x * w, tv.y * w, tv.z * w, float(d_i));
-        s_geo[i] = make_float4(float(r_d), profile_density(P.prof[0], alt),
-                               profile_density(P.prof[1], alt), float(sin_h));
-        s_lut[i] = make_float4(float(cos_h), float(d_min), float(d_max - d_min), tr.w);
-        s_inv[i] = make_float4(float(G(1) / r_d), float(G(1) / (d_max - d_min)),
-                               float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))), 0.0f);
-        s_trow[i] = make_int2(tr.i0 * P.t_mu, tr.i1 * P.t_mu);
-        // The multiple-scattering passes march the same 51 samples of this
-        // row (multiple_scattering.comp:16-47, same dx and d_i): their
-        // per-sample terms are written once here and read by every order.
-        {
-            G mu_i = clamp_cosine<G>((r * mu + d_i) / r_d);
-            Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r_d, mu_i, ground), MU);
-            Taps tz = linear_taps<G>(scattering_u_r<G>(P, r_d), P.s_r);
-            const float f = float(dx) * w;
-            float4* o = MR + (size_t(z) * MU + y) * kMsRowFloat4s;
-            o[i] = make_float4(tv.x * f, tv.y * f, tv.z * f, float(d_i));
-            o[kMsSamples + i] = make_float4(float(G(1) / r_d), ty.w, tz.w, 0.0f);
-            reinterpret_cast<int4*>(o + 2 * kMsSamples)[i] =
-                make_int4((tz.i0 * MU + ty.i0) * W, (tz.i0 * MU + ty.i1) * W,
-                          (tz.i1 * MU + ty.i0) * W, (tz.i1 * MU + ty.i1) * W);
-        }
-    }
+    const float4* row = SR + (size_t(z) * MU + y) * kSsRowFloat4s;
+    for (int e = tid; e < kSsRowFloat4s; e += nt) s_row[e] = row[e];
+    const G r = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z)).r;
     __syncthreads();
     const float rf = float(r);
     const float alpha = P.sun_angular_radius;
@@ -266,7 +281,8 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
         Sum<COMP> rs[3], ms[3];
         for (int i = 0; i < kMsSamples; ++i) {
             const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i];
-            const int2 rows = s_trow[i];
+            const float4 tr4 = s_trw[i];
+            const int2 rows = make_int2(__float_as_int(tr4.x), __float_as_int(tr4.y));
             const float r_d = gq.x, d_i = tv.w;
             Taps tx;
             float mu_s_d;
@@ -306,7 +322,7 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
             ms[1].add(t1 * gq.z);
             ms[2].add(t2 * gq.z);
         }
-        const float dxf = float(dx);
+        const float dxf = s_trw[0].z;
         float3 ray = make_float3(rs[0].value() * dxf * P.solar[0] * P.rayleigh_scattering[0],
                                  rs[1].value() * dxf * P.solar[1] * P.rayleigh_scattering[1],
                                  rs[2].value() * dxf * P.solar[2] * P.rayleigh_scattering[2]);
@@ -517,7 +533,7 @@ __device__ __forceinline__ void density_geometry_block(const DevParams& P, const
                 const G dg = distance_to_bottom<G>(P, r, ct);
                 float3 tg;
                 if constexpr (sizeof(G) == sizeof(double))
-                    tg = get_transmittance_ext(P, OD, r, ct, dg, true);
+                    tg = get_transmittance_precise(P, T, OD, r, ct, dg, true);
                 else
                     tg = get_transmittance<G>(P, T, r, ct, dg, true);
                 gv = make_float4(tg.x * P.albedo[0] * (1.0f / kPi),
@@ -1046,14 +1062,13 @@ __device__ __forceinline__ unsigned logical_block(bool remap) {
 
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const float4* __restrict__ T,
-                                                           const float4* __restrict__ OD,
+                                                           const float4* __restrict__ SR,
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
                                                            float4* __restrict__ S,
-                                                           float4* __restrict__ MR,
                                                            unsigned remap) {
     const unsigned b = logical_block(remap);
-    single_scattering_block<G, COMP>(P, b % P.s_mu, b / P.s_mu, 0, T, OD, dR, dM, S, MR);
+    single_scattering_block<G, COMP>(P, b % P.s_mu, b / P.s_mu, 0, T, SR, dR, dM, S);
 }
 template <typename G, bool COMP>
 __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
@@ -1068,8 +1083,8 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs& l = L[atm];
-        single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.OD, l.dR,
-                                         l.dM, l.S, l.MR);
+        single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.SR, l.dR,
+                                         l.dM, l.S);
     }
 }
 
@@ -1083,21 +1098,39 @@ __global__ __launch_bounds__(256, 8) void k_scattering_density(
                                         dR, dM, dMS, dE, J);
 }
 
+// Per-set geometry: blocks [0, R + MUS) fill the density geometry table,
+// the next MU * R blocks the single- and multiple-scattering row tables.
 template <typename G>
-__global__ __launch_bounds__(64) void k_density_geometry(DevParams P, const float4* __restrict__ T,
-                                                         const float4* __restrict__ OD,
-                                                         float4* __restrict__ DG) {
-    density_geometry_block<G>(P, blockIdx.x, T, OD, DG);
+__device__ __forceinline__ void geometry_block(const DevParams& P, const int b,
+                                               const float4* __restrict__ T,
+                                               const float4* __restrict__ OD,
+                                               float4* __restrict__ DG, float4* __restrict__ SR,
+                                               float4* __restrict__ MR) {
+    const int nd = P.s_r + P.s_mus;
+    if (b < nd) {
+        density_geometry_block<G>(P, b, T, OD, DG);
+    } else {
+        const int rb = b - nd;
+        row_geometry_block<G>(P, rb % P.s_mu, rb / P.s_mu, T, OD, SR, MR);
+    }
 }
 template <typename G>
-__global__ __launch_bounds__(64) void kb_density_geometry(const DevParams* __restrict__ Ps,
-                                                          const LutPtrs* __restrict__ L,
-                                                          const uint32_t* __restrict__ tiles,
-                                                          unsigned ntiles) {
+__global__ __launch_bounds__(64) void k_geometry(DevParams P, const float4* __restrict__ T,
+                                                 const float4* __restrict__ OD,
+                                                 float4* __restrict__ DG, float4* __restrict__ SR,
+                                                 float4* __restrict__ MR) {
+    geometry_block<G>(P, blockIdx.x, T, OD, DG, SR, MR);
+}
+template <typename G>
+__global__ __launch_bounds__(64) void kb_geometry(const DevParams* __restrict__ Ps,
+                                                  const LutPtrs* __restrict__ L,
+                                                  const uint32_t* __restrict__ tiles,
+                                                  unsigned ntiles) {
     for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
         int atm, local;
         decode_tile(tiles[t], atm, local);
-        density_geometry_block<G>(Ps[atm], local, L[atm].T, L[atm].OD, L[atm].DG);
+        const LutPtrs& l = L[atm];
+        geometry_block<G>(Ps[atm], local, l.T, l.OD, l.DG, l.SR, l.MR);
     }
 }
 template <typename G, bool SINGLE>
@@ -1214,26 +1247,26 @@ hipError_t launch_direct_irradiance(const DevParams& P, const float4* T, float4*
     return hipGetLastError();
 }
 
-hipError_t launch_density_geometry(const DevParams& P, const float4* T, const float4* OD,
-                                   float4* DG, hipStream_t s) {
-    const unsigned blocks = unsigned(P.s_r + P.s_mus);
+hipError_t launch_geometry(const DevParams& P, const float4* T, const float4* OD, float4* DG,
+                           float4* SR, float4* MR, hipStream_t s) {
+    const unsigned blocks = unsigned(P.s_r + P.s_mus + P.s_mu * P.s_r);
     if (P.flags & 1)
-        k_density_geometry<double><<<blocks, 64, 0, s>>>(P, T, OD, DG);
+        k_geometry<double><<<blocks, 64, 0, s>>>(P, T, OD, DG, SR, MR);
     else
-        k_density_geometry<float><<<blocks, 64, 0, s>>>(P, T, OD, DG);
+        k_geometry<float><<<blocks, 64, 0, s>>>(P, T, OD, DG, SR, MR);
     return hipGetLastError();
 }
 
-hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* OD,
-                                    float4* dR, float4* dM, float4* S, float4* MR,
-                                    const Tuning& tu, hipStream_t s) {
+hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* SR,
+                                    float4* dR, float4* dM, float4* S, const Tuning& tu,
+                                    hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassSingle, 2, tu);
     const unsigned remap = (tu.remap >> kPassSingle) & 1u;
     dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
-        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, OD, dR, dM, S, MR, remap);
+        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
     else
-        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, OD, dR, dM, S, MR, remap);
+        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
     return hipGetLastError();
 }
 
@@ -1368,8 +1401,8 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
             return {unsigned(P.s_mu) * unsigned(P.s_r), threads, 0};
         case kPassClear:
             return {cdiv(ni, 256), 256, 0};
-        case kPassDensityGeometry:
-            return {unsigned(P.s_r + P.s_mus), 64, 0};
+        case kPassGeometry:
+            return {unsigned(P.s_r + P.s_mus + P.s_mu * P.s_r), 64, 0};
         case kPassDensity:
             return {cdiv(size_t(P.s_mu) * P.s_nu, threads) * unsigned(P.s_mus) * unsigned(P.s_r),
                     threads, density_lds_bytes(P.s_nu, P.i_mus, (order - 1) == 1)};
@@ -1413,11 +1446,11 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
         case kPassClear:
             kb_clear_irradiance<<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
-        case kPassDensityGeometry:
+        case kPassGeometry:
             if (precise)
-                kb_density_geometry<double><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+                kb_geometry<double><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_density_geometry<float><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+                kb_geometry<float><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassDensity:
             if (precise && single)
@@ -1472,9 +1505,8 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
         case kPassClear:
             fn = (const void*)kb_clear_irradiance;
             break;
-        case kPassDensityGeometry:
-            fn = precise ? (const void*)kb_density_geometry<double>
-                         : (const void*)kb_density_geometry<float>;
+        case kPassGeometry:
+            fn = precise ? (const void*)kb_geometry<double> : (const void*)kb_geometry<float>;
             break;
         case kPassDensity:
             fn = precise ? (single ? (const void*)kb_scattering_density<double, true>

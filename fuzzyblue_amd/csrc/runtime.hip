@@ -104,6 +104,7 @@ Sizes sizes_of(const fb_parameters& p) {
     s.nmr = size_t(p.scattering_mu_size) * p.scattering_r_size * fbk::kMsRowFloat4s;
     s.ndg = fbk::dg_count(int(p.scattering_r_size), int(p.scattering_mu_s_size),
                           int(p.scattering_nu_size));
+    s.nsr = size_t(p.scattering_mu_size) * p.scattering_r_size * fbk::kSsRowFloat4s;
     return s;
 }
 
@@ -169,6 +170,7 @@ void free_pending_scratch(fb_pending_atmosphere* pa) {
     free_buf(b, &pa->MR);
     free_buf(b, &pa->OD);
     free_buf(b, &pa->DG);
+    free_buf(b, &pa->SR);
 }
 
 void free_atmosphere_buffers(fb_atmosphere* a) {
@@ -202,6 +204,7 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
     if (st == FB_OK) st = alloc_buf(b, s.nmr, &pa->MR);
     if (st == FB_OK) st = alloc_buf(b, s.nt, &pa->OD);
     if (st == FB_OK) st = alloc_buf(b, s.ndg, &pa->DG);
+    if (st == FB_OK) st = alloc_buf(b, s.nsr, &pa->SR);
     hipError_t e = hipSuccess;
     if (st == FB_OK) e = hipEventCreateWithFlags(&pa->done, hipEventDisableTiming);
     if (st == FB_OK && e == hipSuccess) e = hipEventCreateWithFlags(&a->ready, hipEventDisableTiming);
@@ -262,12 +265,12 @@ fb_status enqueue_precompute(fb_atmosphere* a, fb_pending_atmosphere* pa, hipStr
     float4* OD = (a->params.flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr;
     FB_HIP_TRY(fbk::launch_transmittance(P, a->T.p, OD, s));
     FB_MARK(FB_PASS_TRANSMITTANCE, 1);
-    FB_HIP_TRY(fbk::launch_density_geometry(P, a->T.p, OD, pa->DG.p, s));
-    FB_MARK(FB_PASS_DENSITY_GEOMETRY, 1);
+    FB_HIP_TRY(fbk::launch_geometry(P, a->T.p, OD, pa->DG.p, pa->SR.p, pa->MR.p, s));
+    FB_MARK(FB_PASS_GEOMETRY, 1);
     FB_HIP_TRY(fbk::launch_direct_irradiance(P, a->T.p, pa->dE.p, s));
     FB_MARK(FB_PASS_DIRECT_IRRADIANCE, 1);
     FB_HIP_TRY(
-        fbk::launch_single_scattering(P, a->T.p, OD, pa->dR.p, pa->dM.p, a->S.p, pa->MR.p, tu, s));
+        fbk::launch_single_scattering(P, a->T.p, pa->SR.p, pa->dR.p, pa->dM.p, a->S.p, tu, s));
     FB_MARK(FB_PASS_SINGLE_SCATTERING, 1);
     FB_HIP_TRY(fbk::launch_fill_zero(a->E.p, sz.ni, s));
     FB_MARK(FB_PASS_CLEAR_IRRADIANCE, 1);
@@ -461,7 +464,7 @@ size_t fb_parameters_footprint(const fb_parameters* p, size_t* resident, size_t*
     if (!p) return 0;
     Sizes s = sizes_of(*p);
     size_t res = (s.nt + s.ni + s.ns) * sizeof(float4);
-    size_t scr = (s.nt + s.ni + 4 * s.ns + s.nmr + s.ndg) * sizeof(float4);
+    size_t scr = (s.nt + s.ni + 4 * s.ns + s.nmr + s.ndg + s.nsr) * sizeof(float4);
     if (resident) *resident = res;
     if (scratch) *scratch = scr;
     return res + scr;

@@ -59,6 +59,7 @@ struct Sizes {
     size_t nt, ni, ns;  // texel counts: transmittance, irradiance, scattering
     size_t nmr;         // float4s of the multiple-scattering row table
     size_t ndg;         // float4s of the density geometry table
+    size_t nsr;         // float4s of the single-scattering row table
 };
 Sizes sizes_of(const fb_parameters& p);
 
@@ -107,7 +108,7 @@ struct fb_atmosphere {
 struct fb_pending_atmosphere {
     fb_builder* builder = nullptr;
     fb_atmosphere* atm = nullptr;
-    fbrt::Buf dE, dR, dM, dMS, J, MR, OD, DG;
+    fbrt::Buf dE, dR, dM, dMS, J, MR, OD, DG, SR;
     hipEvent_t done = nullptr;
     hipGraphExec_t graph = nullptr;
     std::shared_ptr<void> batch;  // shared state of a batched build

@@ -149,12 +149,13 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
            *MR = pa->MR.p;
     const float4* OD = (p->flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr;
     const float4* DG = pa->DG.p;
+    const float4* SR = pa->SR.p;
     // (The repeated launches rewrite the scratch set's LUTs, and the
     // indirect and multiple passes accumulate into E and S; the set is
     // discarded afterwards.)
     st = tune(0, fbk::kPassSingle, 1, &fbk::Tuning::single, 1, fbk::kPassSingle,
               [&](const fbk::Tuning& t) {
-        return fbk::launch_single_scattering(P, T, OD, dR, dM, S, MR, t, s);
+        return fbk::launch_single_scattering(P, T, SR, dR, dM, S, t, s);
     });
     if (st == FB_OK)
         st = tune(1, fbk::kPassDensity, 2, &fbk::Tuning::density2, 1, -1,

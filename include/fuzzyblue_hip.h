@@ -208,11 +208,12 @@ typedef enum fb_pass {
     FB_PASS_SCATTERING_DENSITY = 4,  /* shaders/scattering_density.comp */
     FB_PASS_INDIRECT_IRRADIANCE = 5, /* shaders/indirect_irradiance.comp */
     FB_PASS_MULTIPLE_SCATTERING = 6, /* shaders/multiple_scattering.comp */
-    FB_PASS_DENSITY_GEOMETRY = 7     /* no reference counterpart: per-set
-                                        table of the density pass's taps and
-                                        ground terms (scattering_density.comp
-                                        :34-66), built once after the
-                                        transmittance pass */
+    FB_PASS_GEOMETRY = 7             /* no reference counterpart: per-set
+                                        geometry tables built once after the
+                                        transmittance pass: the density
+                                        pass's taps and ground terms, and the
+                                        51 samples of every (r, mu) row that
+                                        single and multiple scattering march */
 } fb_pass;
 
 typedef struct fb_pass_timing {
