@@ -215,6 +215,113 @@ def measure_batch(b, stream, n=256, seed=4):
     }
 
 
+def _rel(x, y, scale):
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    return np.abs(x - y) / np.maximum(np.abs(y), 1e-6 * scale)
+
+
+def measure_precision(b, stream):
+    """The precision workload (Earth, LUTs at twice the default resolution,
+    order 8) in precision mode against its FP64-oracle fixture: max / mean
+    relative error per LUT, and per scattering order, for texels whose view
+    cosine is within 1e-3 of the horizon and for the rest."""
+    path = os.path.join(ROOT, "tests", "golden", "earth_double_order8_fp64.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    band = g["S_band"]
+
+    def split(e, m):
+        return {"horizon_max": float(e[m].max()), "horizon_mean": float(e[m].mean()),
+                "other_max": float(e[~m].max()), "other_mean": float(e[~m].mean())}
+
+    def build(order):
+        pa = Atmosphere.build(b, stream, Parameters().replace(
+            **RESOLUTIONS["double"], order=order, flags=abi.FB_FLAG_PRECISE))
+        pa.wait()
+        return pa
+
+    pa = build(8)
+    a = pa.atmosphere()
+    T = a.download("transmittance")[..., :3]
+    E = a.download("irradiance")[..., :3]
+    S = a.download("scattering").reshape(-1, 4)[g["S_idx"]]
+    pa.close()
+    out = {
+        "workload": "Earth, transmittance 512x128, irradiance 128x32, scattering 64x256x64x16, "
+                    "order 8, precision mode",
+        "transmittance_max": float(_rel(T, g["T"], np.abs(g["T"]).max()).max()),
+        "irradiance_max": float(_rel(E, g["E"], np.abs(g["E"]).max()).max()),
+        "scattering": split(_rel(S[:, :3], g["S"][:, :3],
+                                 np.abs(g["S"][:, :3]).max(axis=0)).max(axis=1), band),
+        "per_order": [],
+    }
+    sel, oband = g["S_idx"][g["O_sel"]], band[g["O_sel"]]
+    for order in range(1, 9):
+        pa = build(order)
+        which = "delta_rayleigh" if order == 1 else "delta_multiple_scattering"
+        s_o = pa.download_scratch(which).reshape(-1, 4)[sel, :3]
+        e_o = pa.download_scratch("delta_irradiance")[..., :3]
+        pa.close()
+        ref_s, ref_e = g["orders_S"][order - 1], g["orders_E"][order - 1]
+        out["per_order"].append({
+            "order": order,
+            "scattering": split(_rel(s_o, ref_s, np.abs(ref_s).max(axis=0)).max(axis=1), oband),
+            "irradiance_max": float(_rel(e_o, ref_e, np.abs(ref_e).max()).max())})
+    return out
+
+
+def measure_mixed_accuracy(b, stream):
+    """Eight random atmospheres drawn like the mixed set (half resolution,
+    orders 2-8) as one precision-mode batch against their FP64-oracle LUTs:
+    the worst member's max relative error per LUT."""
+    import json as _json
+
+    path = os.path.join(ROOT, "tests", "golden", "mixed_half_fp64.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    ps = [Parameters.from_dict(d) for d in _json.loads(str(g["params"]))]
+    pend = Atmosphere.build_batch(b, stream, ps)
+    worst = {"transmittance": 0.0, "irradiance": 0.0, "scattering": 0.0}
+    for i, pa in enumerate(pend):
+        pa.wait()
+        a = pa.atmosphere()
+        for name, key, x in (("transmittance", "T", a.download("transmittance")[..., :3]),
+                             ("irradiance", "E", a.download("irradiance")[..., :3]),
+                             ("scattering", "S",
+                              a.download("scattering").reshape(-1, 4)[g["S_idx"]])):
+            y = g[f"{key}{i}"]
+            scale = np.abs(y).reshape(-1, y.shape[-1]).max(axis=0)
+            worst[name] = max(worst[name], float(_rel(x, y, scale).max()))
+    for pa in pend:
+        pa.close()
+    return {"atmospheres": len(ps), "max_rel_err": worst}
+
+
+def measure_served_mixed(b, n=256, seed=4, budget_gb=200.0):
+    """The whole seeded mixed set submitted to the serving loop at once:
+    LUT sets per second from first submission to the last completion."""
+    from fuzzyblue_amd.serving import Server
+    from fuzzyblue_amd.workloads import mixed_batch
+
+    ps = mixed_batch(n, seed, precise=True)
+    srv = Server(b, int(budget_gb * 1e9), max_inflight=64, n_streams=4)
+    t0 = time.perf_counter()
+    for p in ps:
+        srv.submit_precompute(p)
+    srv.drain()
+    wall = time.perf_counter() - t0
+    infos = srv.poll()
+    lat = np.array([1e3 * (i.t_done - i.t_submit) for i in infos if i.status == abi.FB_OK])
+    failed = sum(1 for i in infos if i.status != abi.FB_OK)
+    srv.close()
+    return {"atmospheres": n, "lut_sets_s": float(lat.size / wall), "wall_s": wall,
+            "failed": int(failed), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99))}
+
+
 def measure_serving(b, rates, duration, budget_gb=200.0):
     """Poisson arrivals (workloads.arrivals: 20% precompute drawn like the
     mixed set, 80% sky renders of 640x360 / 1920x1080 / 3840x2160 rays
@@ -302,6 +409,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--serving-rates", default="50",
                     help="comma-separated Poisson rates (req/s) for the serving run; "
                          "empty to skip")
@@ -377,7 +485,16 @@ def main():
     if not args.no_batch:
         log("[bench] mixed batch of 256 atmospheres")
         batch = measure_batch(b, stream)
+        batch["served"] = measure_served_mixed(b)
+        batch["served_speedup_vs_plain"] = (batch["served"]["lut_sets_s"] /
+                                            batch["plain_lut_sets_s"])
         log(f"[bench] batch: {batch}")
+
+    precision = mixed = None
+    if not args.no_precision:
+        log("[bench] precision workload and mixed-set accuracy")
+        precision = measure_precision(b, stream)
+        mixed = measure_mixed_accuracy(b, stream)
 
     serving = None
     if args.serving_rates:
@@ -422,6 +539,8 @@ def main():
         "sky": sky,
         "batch_mixed256": batch,
         "serving": serving,
+        "precision_double_order8": precision,
+        "mixed_half_vs_fp64": mixed,
         "roofline": {
             "bound": "valu",
             "kernel": "scattering_density",
