@@ -32,6 +32,7 @@ struct BatchLaunch {
     std::vector<uint32_t> host_tiles;  // until uploaded
     size_t offset = 0;                 // of d_tiles in the plan's device block
     unsigned grid = 0;                 // persistent grid size (0: one block per tile)
+    unsigned tpt = 1;                  // multiple scattering: texels per thread
 };
 
 // Shared state of one batch: device arrays, the launch list and the graph.
@@ -66,7 +67,7 @@ struct BatchPlan {
     fb_status enqueue(hipStream_t s) const {
         for (const auto& l : launches)
             FB_HIP_TRY(fbk::launch_batch(l.pass, l.order, precise, d_params, d_luts, l.d_tiles,
-                                         l.ntiles, l.threads, l.lds, l.grid, s));
+                                         l.ntiles, l.threads, l.lds, l.grid, l.tpt, s));
         return FB_OK;
     }
 };
@@ -83,16 +84,34 @@ std::vector<int> tile_order(const std::vector<fb_parameters>& ps) {
     return idx;
 }
 
+// Texels per thread of a multiple-scattering row of W texels at `threads`
+// threads per block (the kernel variants 1, 2, 4).
+unsigned ms_tpt(const fbk::DevParams& P, unsigned threads) {
+    const unsigned W = unsigned(P.s_nu * P.s_mus);
+    return W <= threads ? 1u : (W <= 2 * threads ? 2u : 4u);
+}
+
+// XCD-contiguous order of a tile list: hardware block i runs on XCD i % 8,
+// so tile i of the permuted list is logical tile xcd_remap(i), and XCD k
+// covers the k-th contiguous eighth of the logical list.
+std::vector<uint32_t> xcd_order(const std::vector<uint32_t>& tiles) {
+    const size_t n = tiles.size(), per = n / 8, rem = n % 8;
+    std::vector<uint32_t> out(n);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t xcd = i % 8, local = i / 8;
+        out[i] = tiles[xcd * per + std::min(xcd, rem) + local];
+    }
+    return out;
+}
+
 fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
                      const std::vector<fbk::DevParams>& dps, const std::vector<fbk::Tuning>& tus,
                      const std::vector<int>& order_idx, int pass, int order) {
-    BatchLaunch l;
-    l.pass = pass;
-    l.order = order;
     auto member = [&](int i) { return pass < fbk::kPassDensity || int(ps[i].order) >= order; };
     // Block size of the tunable passes: the tuning of the launch's largest
     // atmosphere when it suits every member, else a size that suits all.
     unsigned threads = 0;
+    bool remap = false;
     if (pass == fbk::kPassSingle || pass == fbk::kPassDensity || pass == fbk::kPassIndirect ||
         pass == fbk::kPassMultiple) {
         int first = -1;
@@ -103,35 +122,49 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
             }
         if (first < 0) return FB_OK;
         threads = fbk::tuned_threads(dps[first], pass, order, tus[first]);
+        remap = (tus[first].remap >> pass) & 1u;
         for (int i : order_idx)
             if (member(i) && !fbk::tuning_valid(dps[i], pass, order, threads))
                 threads = pass == fbk::kPassIndirect ? 64 : 256;
     }
-    std::vector<uint32_t> tiles;
-    for (int i : order_idx) {
-        if (!member(i)) continue;
-        fbk::PassShape sh = fbk::batch_pass_shape(dps[i], pass, order, threads);
-        if (sh.blocks >= (1u << 24)) {
-            set_error("atmosphere %d needs %u blocks in pass %d (limit 2^24)", i, sh.blocks, pass);
+    // Multiple scattering: one launch per texels-per-thread class, so rows
+    // that fit one texel per thread run the kernel variant that overlaps
+    // its loads with evaluation.  Other passes: one launch.
+    const unsigned classes[] = {1u, 2u, 4u};
+    for (unsigned cls : classes) {
+        if (pass != fbk::kPassMultiple && cls != 1u) break;
+        BatchLaunch l;
+        l.pass = pass;
+        l.order = order;
+        l.tpt = pass == fbk::kPassMultiple ? cls : 1u;
+        std::vector<uint32_t> tiles;
+        for (int i : order_idx) {
+            if (!member(i)) continue;
+            if (pass == fbk::kPassMultiple && ms_tpt(dps[i], threads) != cls) continue;
+            fbk::PassShape sh = fbk::batch_pass_shape(dps[i], pass, order, threads);
+            if (sh.blocks >= (1u << 24)) {
+                set_error("atmosphere %d needs %u blocks in pass %d (limit 2^24)", i, sh.blocks,
+                          pass);
+                return FB_EINVAL;
+            }
+            for (unsigned b = 0; b < sh.blocks; ++b) tiles.push_back((uint32_t(i) << 24) | b);
+            l.threads = std::max(l.threads, sh.threads);
+            l.lds = std::max(l.lds, sh.lds);
+        }
+        if (tiles.empty()) continue;
+        if (l.lds > 65536) {
+            set_error("pass %d needs %zu bytes of LDS per block", pass, l.lds);
             return FB_EINVAL;
         }
-        for (unsigned b = 0; b < sh.blocks; ++b) tiles.push_back((uint32_t(i) << 24) | b);
-        l.threads = std::max(l.threads, sh.threads);
-        l.lds = std::max(l.lds, sh.lds);
+        l.ntiles = unsigned(tiles.size());
+        if (plan.share_pct < 100) {
+            const unsigned resident =
+                fbk::batch_resident_blocks(pass, order, plan.precise, l.threads, l.lds, l.tpt);
+            if (resident) l.grid = std::max(1u, resident * plan.share_pct / 100);
+        }
+        l.host_tiles = remap ? xcd_order(tiles) : std::move(tiles);
+        plan.launches.push_back(std::move(l));
     }
-    if (tiles.empty()) return FB_OK;
-    if (l.lds > 65536) {
-        set_error("pass %d needs %zu bytes of LDS per block", pass, l.lds);
-        return FB_EINVAL;
-    }
-    l.ntiles = unsigned(tiles.size());
-    if (plan.share_pct < 100) {
-        const unsigned resident =
-            fbk::batch_resident_blocks(pass, order, plan.precise, l.threads, l.lds);
-        if (resident) l.grid = std::max(1u, resident * plan.share_pct / 100);
-    }
-    l.host_tiles = std::move(tiles);
-    plan.launches.push_back(std::move(l));
     return FB_OK;
 }
 
@@ -239,7 +272,7 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
         for (size_t li = 0; li < plan->launches.size(); ++li) {
             const BatchLaunch& l = plan->launches[li];
             e = fbk::launch_batch(l.pass, l.order, plan->precise, plan->d_params, plan->d_luts,
-                                  l.d_tiles, l.ntiles, l.threads, l.lds, l.grid, s);
+                                  l.d_tiles, l.ntiles, l.threads, l.lds, l.grid, l.tpt, s);
             for (size_t i = 0; i < n && e == hipSuccess; ++i) {
                 if (last[i] != li) continue;
                 e = hipEventRecord(plan->members[i]->done, s);

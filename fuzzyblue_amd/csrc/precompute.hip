@@ -1185,7 +1185,7 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
     const unsigned b = logical_block(remap);
     multiple_scattering_block<G, COMP, TPT>(P, b % P.s_mu, b / P.s_mu, 0, MR, J, dMS, S);
 }
-template <typename G, bool COMP>
+template <typename G, bool COMP, int TPT>
 __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
                                                               const LutPtrs* __restrict__ L,
                                                               const uint32_t* __restrict__ tiles, unsigned ntiles) {
@@ -1198,8 +1198,8 @@ __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* _
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs& l = L[atm];
-        multiple_scattering_block<G, COMP, kMsTexelsPerThread>(P, local % P.s_mu, local / P.s_mu, 0,
-                                                              l.MR, l.J, l.dMS, l.S);
+        multiple_scattering_block<G, COMP, TPT>(P, local % P.s_mu, local / P.s_mu, 0, l.MR, l.J,
+                                                l.dMS, l.S);
     }
 }
 
@@ -1420,7 +1420,8 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
 
 hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
                         const LutPtrs* L, const uint32_t* tiles, unsigned ntiles,
-                        unsigned threads, size_t lds, unsigned grid_blocks, hipStream_t s) {
+                        unsigned threads, size_t lds, unsigned grid_blocks, unsigned tpt,
+                        hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
     const dim3 grid(grid_blocks && grid_blocks < ntiles ? grid_blocks : ntiles), block(threads);
     const bool single = (order - 1) == 1;
@@ -1473,10 +1474,12 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
                 kb_indirect_irradiance<float, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassMultiple:
+#define FB_KBMS(G, T) kb_multiple_scattering<G, false, T><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles)
             if (precise)
-                kb_multiple_scattering<double, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                tpt == 1 ? FB_KBMS(double, 1) : tpt == 2 ? FB_KBMS(double, 2) : FB_KBMS(double, 4);
             else
-                kb_multiple_scattering<float, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                tpt == 1 ? FB_KBMS(float, 1) : tpt == 2 ? FB_KBMS(float, 2) : FB_KBMS(float, 4);
+#undef FB_KBMS
             break;
         default:
             return hipErrorInvalidValue;
@@ -1486,7 +1489,8 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
 
 // Blocks of one batched launch that fit on the device at once (per-CU
 // occupancy of the launch's kernel x CUs); 0 if it cannot be determined.
-unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threads, size_t lds) {
+unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threads, size_t lds,
+                               unsigned tpt) {
     const bool single = (order - 1) == 1;
     const void* fn = nullptr;
     switch (pass) {
@@ -1521,8 +1525,15 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
                                    : (const void*)kb_indirect_irradiance<float, false>);
             break;
         case kPassMultiple:
-            fn = precise ? (const void*)kb_multiple_scattering<double, false>
-                         : (const void*)kb_multiple_scattering<float, false>;
+            if (tpt == 1)
+                fn = precise ? (const void*)kb_multiple_scattering<double, false, 1>
+                             : (const void*)kb_multiple_scattering<float, false, 1>;
+            else if (tpt == 2)
+                fn = precise ? (const void*)kb_multiple_scattering<double, false, 2>
+                             : (const void*)kb_multiple_scattering<float, false, 2>;
+            else
+                fn = precise ? (const void*)kb_multiple_scattering<double, false, 4>
+                             : (const void*)kb_multiple_scattering<float, false, 4>;
             break;
     }
     int per_cu = 0, dev = 0, cus = 0;
