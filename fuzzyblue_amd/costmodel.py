@@ -80,10 +80,10 @@ def pass_work(p, name: str, order: int = 2) -> Work:
     if name == "geometry":
         # per-set tables: the density pass's taps and ground terms (33 texels
         # per r, NU + 2 per mu_s) and the 51 row samples of single (5
-        # texels each) and multiple (3 texels each) scattering
+        # texels each) and multiple (4 texels each) scattering
         r, mus, nu = p.scattering_r_size, p.scattering_mu_s_size, p.scattering_nu_size
         rows = p.scattering_r_size * p.scattering_mu_size
-        return Work(0, 0, (r * 33 + mus * (nu + 2) + rows * 51 * 8) * b)
+        return Work(0, 0, (r * 33 + mus * (nu + 2) + rows * 51 * 9) * b)
     if name == "scattering_density":
         key = "scattering_density_2" if order == 2 else "scattering_density_n"
         s, f, x = PASS_COST[key]

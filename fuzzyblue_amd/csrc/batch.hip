@@ -108,40 +108,49 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
                      const std::vector<fbk::DevParams>& dps, const std::vector<fbk::Tuning>& tus,
                      const std::vector<int>& order_idx, int pass, int order) {
     auto member = [&](int i) { return pass < fbk::kPassDensity || int(ps[i].order) >= order; };
-    // Block size of the tunable passes: the tuning of the launch's largest
-    // atmosphere when it suits every member, else a size that suits all.
-    unsigned threads = 0;
-    bool remap = false;
-    if (pass == fbk::kPassSingle || pass == fbk::kPassDensity || pass == fbk::kPassIndirect ||
-        pass == fbk::kPassMultiple) {
-        int first = -1;
-        for (int i : order_idx)
-            if (member(i)) {
-                first = i;
-                break;
-            }
-        if (first < 0) return FB_OK;
-        threads = fbk::tuned_threads(dps[first], pass, order, tus[first]);
-        remap = (tus[first].remap >> pass) & 1u;
-        for (int i : order_idx)
-            if (member(i) && !fbk::tuning_valid(dps[i], pass, order, threads))
-                threads = pass == fbk::kPassIndirect ? 64 : 256;
+    // One launch per block shape.  Multiple scattering: each atmosphere runs
+    // its own tuned shape (rows of very different widths want different
+    // block sizes), one launch per distinct (threads, texels per thread).
+    // The other tunable passes: one launch, with the tuning of its largest
+    // atmosphere when that suits every member, else a size that suits all.
+    struct Group {
+        unsigned threads, tpt;
+        bool remap;
+        std::vector<int> members;
+    };
+    std::vector<Group> groups;
+    const bool tunable = pass == fbk::kPassSingle || pass == fbk::kPassDensity ||
+                         pass == fbk::kPassIndirect || pass == fbk::kPassMultiple;
+    for (int i : order_idx) {
+        if (!member(i)) continue;
+        if (pass == fbk::kPassMultiple) {
+            const unsigned t = fbk::tuned_threads(dps[i], pass, order, tus[i]);
+            const unsigned c = ms_tpt(dps[i], t);
+            auto g = std::find_if(groups.begin(), groups.end(),
+                                  [&](const Group& x) { return x.threads == t && x.tpt == c; });
+            if (g == groups.end())
+                groups.push_back(Group{t, c, bool((tus[i].remap >> pass) & 1u), {i}});
+            else
+                g->members.push_back(i);
+        } else if (groups.empty()) {
+            groups.push_back(Group{tunable ? fbk::tuned_threads(dps[i], pass, order, tus[i]) : 0u,
+                                   1u, tunable && ((tus[i].remap >> pass) & 1u), {i}});
+        } else {
+            groups[0].members.push_back(i);
+        }
     }
-    // Multiple scattering: one launch per texels-per-thread class, so rows
-    // that fit one texel per thread run the kernel variant that overlaps
-    // its loads with evaluation.  Other passes: one launch.
-    const unsigned classes[] = {1u, 2u, 4u};
-    for (unsigned cls : classes) {
-        if (pass != fbk::kPassMultiple && cls != 1u) break;
+    if (tunable && pass != fbk::kPassMultiple && !groups.empty())
+        for (int i : groups[0].members)
+            if (!fbk::tuning_valid(dps[i], pass, order, groups[0].threads))
+                groups[0].threads = pass == fbk::kPassIndirect ? 64 : 256;
+    for (Group& g : groups) {
         BatchLaunch l;
         l.pass = pass;
         l.order = order;
-        l.tpt = pass == fbk::kPassMultiple ? cls : 1u;
+        l.tpt = g.tpt;
         std::vector<uint32_t> tiles;
-        for (int i : order_idx) {
-            if (!member(i)) continue;
-            if (pass == fbk::kPassMultiple && ms_tpt(dps[i], threads) != cls) continue;
-            fbk::PassShape sh = fbk::batch_pass_shape(dps[i], pass, order, threads);
+        for (int i : g.members) {
+            fbk::PassShape sh = fbk::batch_pass_shape(dps[i], pass, order, g.threads);
             if (sh.blocks >= (1u << 24)) {
                 set_error("atmosphere %d needs %u blocks in pass %d (limit 2^24)", i, sh.blocks,
                           pass);
@@ -162,7 +171,7 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
                 fbk::batch_resident_blocks(pass, order, plan.precise, l.threads, l.lds, l.tpt);
             if (resident) l.grid = std::max(1u, resident * plan.share_pct / 100);
         }
-        l.host_tiles = remap ? xcd_order(tiles) : std::move(tiles);
+        l.host_tiles = g.remap ? xcd_order(tiles) : std::move(tiles);
         plan.launches.push_back(std::move(l));
     }
     return FB_OK;

@@ -24,9 +24,10 @@ constexpr int kMaxRowTexels = 1024;  // NU * MUS: one scattering row per block
 // SAMPLE_COUNT + 1 (single_scattering.comp:42, multiple_scattering.comp:21).
 constexpr int kMsSamples = 51;
 // Multiple-scattering row table: per (r, mu) row and sample, the view
-// transmittance x trapezoid weight (rgb) and d_i, then (1/r_i, mu weight, r
-// weight), then the offsets of the four (r, mu) density rows.
-constexpr int kMsRowFloat4s = 3 * kMsSamples;
+// transmittance x dx x trapezoid weight (rgb) and d_i; (1/r_i, mask of the
+// density-row slots to load); the row offsets loaded into those slots; the
+// slots' bilinear weights.
+constexpr int kMsRowFloat4s = 4 * kMsSamples;
 // Single-scattering row table: per (r, mu) row and sample, the view
 // transmittance x trapezoid weight (rgb) and d_i; (r_d, Rayleigh density,
 // Mie density, sin of the horizon angle); (cos of the horizon angle,

@@ -11,6 +11,16 @@
 #include "atmo_device.hpp"
 #include "kernels.hpp"
 
+// The constant address space (read-only for the whole kernel): loads through
+// it are scalar when uniform, and addresses loaded from it are known to
+// point to global memory.  The host pass only parses device code, without
+// address spaces.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FB_CONSTANT_AS __attribute__((address_space(4)))
+#else
+#define FB_CONSTANT_AS
+#endif
+
 namespace fbk {
 
 // ------------------------------------------------------------------ tables
@@ -193,6 +203,54 @@ __device__ __forceinline__ void direct_irradiance_block(const DevParams& P, cons
 // then evaluates only its sun transmittance T(r_d, mu_s_d) (one bilinear
 // lookup) and accumulates.
 
+// Multiple scattering interpolates each sample's value from four density
+// rows, (r0, mu0), (r0, mu1), (r1, mu0), (r1, mu1).  Consecutive samples of
+// a ray often share rows (the same cell, or the next one in mu or r: at
+// default sizes 79 distinct rows serve the 204 taps of an (r, mu) row), so
+// the kernel keeps four row "slots" in registers from sample to sample.
+// Row (r index z, mu index y) always lives in slot 2*(z & 1) + (y & 1):
+// the rows of one sample differ by at most one step in z and in y, so they
+// never collide (clamped duplicates are the same row), and a row shared by
+// consecutive samples stays where it is.  Per sample, the table gives the
+// slots to load (a used slot is loaded unless the previous sample used it
+// for the same row), the row offsets to load, and each slot's weight: the
+// summed bilinear weights of its taps, 0 for an unused slot.
+struct MsSlots {
+    int mask;   // bit s: load slot s
+    int4 load;  // row offset loaded into slot s (0 when not loaded)
+    float4 w;   // slot weights
+};
+// i: this lane's sample; z[k], y[k], off[k], wt[k]: its taps (A, B, C, D).
+// Every lane of the wavefront calls this (lane i holds sample i; lanes past
+// the last sample repeat it).
+__device__ __forceinline__ MsSlots ms_row_slots(int i, const int z[4], const int y[4],
+                                                const int off[4], const float wt[4]) {
+    int row[4] = {-1, -1, -1, -1};  // row offset held by each slot, -1 if unused
+    float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int s = ((z[k] & 1) << 1) | (y[k] & 1);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (t == s) {
+                row[t] = off[k];
+                w[t] += wt[k];
+            }
+    }
+    MsSlots r{0, make_int4(0, 0, 0, 0), make_float4(w[0], w[1], w[2], w[3])};
+    int ld[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int prev = __shfl_up(row[t], 1, 64);  // sample i - 1's slot t
+        if (row[t] >= 0 && (i == 0 || prev != row[t])) {
+            r.mask |= 1 << t;
+            ld[t] = row[t];
+        }
+    }
+    r.load = make_int4(ld[0], ld[1], ld[2], ld[3]);
+    return r;
+}
+
 // The single- and multiple-scattering row tables of row (y, z): thread i
 // computes sample i.  The multiple-scattering passes march the same 51
 // samples (multiple_scattering.comp:16-47, the same dx and d_i).
@@ -203,13 +261,15 @@ __device__ __forceinline__ void row_geometry_block(const DevParams& P, const int
                                                    float4* __restrict__ SR,
                                                    float4* __restrict__ MR) {
     const int tid = threadIdx.x;
-    if (tid >= kMsSamples) return;
     const int MU = P.s_mu, W = P.s_nu * P.s_mus;
     const TexelCoord<G> rc = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z));
     const G r = rc.r, mu = rc.mu;
     const bool ground = rc.ground;
     const G dx = distance_to_nearest<G>(P, r, mu, ground) / G(kMsSamples - 1);
-    const int i = tid;
+    // Lanes 51..63 evaluate sample 50 again and write nothing: every lane
+    // takes part in the cross-lane reads of ms_row_slots.
+    const int i = min(tid, kMsSamples - 1);
+    const bool writer = tid < kMsSamples;
     const G d_i = G(i) * dx;
     const G r_d = clamp_radius<G>(P, sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r));
     float3 tv;
@@ -226,15 +286,19 @@ __device__ __forceinline__ void row_geometry_block(const DevParams& P, const int
     const G rho = safe_sqrt<G>(r_d * r_d - G(P.bottom) * G(P.bottom));
     const G d_min = G(P.top) - r_d, d_max = rho + H;
     const Taps tr = linear_taps<G>(tex_from_unit<G>(rho / H, P.t_r), P.t_r);
-    float4* o = SR + (size_t(z) * MU + y) * kSsRowFloat4s;
-    o[i] = make_float4(tv.x * w, tv.y * w, tv.z * w, float(d_i));
-    o[kMsSamples + i] = make_float4(float(r_d), profile_density(P.prof[0], alt),
-                                    profile_density(P.prof[1], alt), float(sin_h));
-    o[2 * kMsSamples + i] = make_float4(float(cos_h), float(d_min), float(d_max - d_min), tr.w);
-    o[3 * kMsSamples + i] = make_float4(float(G(1) / r_d), float(G(1) / (d_max - d_min)),
-                                        float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))), 0.0f);
-    o[4 * kMsSamples + i] = make_float4(__int_as_float(tr.i0 * P.t_mu),
-                                        __int_as_float(tr.i1 * P.t_mu), float(dx), 0.0f);
+    if (writer) {
+        float4* o = SR + (size_t(z) * MU + y) * kSsRowFloat4s;
+        o[i] = make_float4(tv.x * w, tv.y * w, tv.z * w, float(d_i));
+        o[kMsSamples + i] = make_float4(float(r_d), profile_density(P.prof[0], alt),
+                                        profile_density(P.prof[1], alt), float(sin_h));
+        o[2 * kMsSamples + i] =
+            make_float4(float(cos_h), float(d_min), float(d_max - d_min), tr.w);
+        o[3 * kMsSamples + i] =
+            make_float4(float(G(1) / r_d), float(G(1) / (d_max - d_min)),
+                        float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))), 0.0f);
+        o[4 * kMsSamples + i] = make_float4(__int_as_float(tr.i0 * P.t_mu),
+                                            __int_as_float(tr.i1 * P.t_mu), float(dx), 0.0f);
+    }
     // Multiple scattering: T(r, mu, d_i) x dx x weight, then the (mu, r)
     // taps of the sample point (r_i, mu_i).
     const G mu_i = clamp_cosine<G>((r * mu + d_i) / r_d);
@@ -242,11 +306,21 @@ __device__ __forceinline__ void row_geometry_block(const DevParams& P, const int
     const Taps tz = linear_taps<G>(scattering_u_r<G>(P, r_d), P.s_r);
     const float f = float(dx) * w;
     float4* m = MR + (size_t(z) * MU + y) * kMsRowFloat4s;
+    // The four density rows of the sample, (r0, mu0), (r0, mu1), (r1, mu0),
+    // (r1, mu1), as offsets into J, and their bilinear weights.
+    const int tz4[4] = {tz.i0, tz.i0, tz.i1, tz.i1}, ty4[4] = {ty.i0, ty.i1, ty.i0, ty.i1};
+    int off[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) off[k] = (tz4[k] * MU + ty4[k]) * W;
+    const G wy = G(ty.w), wz = G(tz.w);
+    const float wt[4] = {float((G(1) - wy) * (G(1) - wz)), float(wy * (G(1) - wz)),
+                         float((G(1) - wy) * wz), float(wy * wz)};
+    const MsSlots sl = ms_row_slots(i, tz4, ty4, off, wt);
+    if (!writer) return;
     m[i] = make_float4(tv.x * f, tv.y * f, tv.z * f, float(d_i));
-    m[kMsSamples + i] = make_float4(float(G(1) / r_d), ty.w, tz.w, 0.0f);
-    reinterpret_cast<int4*>(m + 2 * kMsSamples)[i] =
-        make_int4((tz.i0 * MU + ty.i0) * W, (tz.i0 * MU + ty.i1) * W, (tz.i1 * MU + ty.i0) * W,
-                  (tz.i1 * MU + ty.i1) * W);
+    m[kMsSamples + i] = make_float4(float(G(1) / r_d), __int_as_float(sl.mask), 0.0f, 0.0f);
+    reinterpret_cast<int4*>(m + 2 * kMsSamples)[i] = sl.load;
+    m[3 * kMsSamples + i] = sl.w;
 }
 
 template <typename G, bool COMP>
@@ -823,6 +897,12 @@ __device__ __forceinline__ void indirect_irradiance_block(const DevParams& P, co
 // row of the density LUT in LDS (double buffered, one barrier per sample);
 // each texel then lerps 2 nu slices x 2 mu_s taps from LDS.
 constexpr int kMsTexelsPerThread = 4;
+// Threads per block of the multiple-scattering kernels: up to one per
+// texel of the widest row (NU * MUS <= 1024).
+constexpr int kMsMaxThreads = 1024;
+// Staged density rows per block (sample i is evaluated from one while
+// sample i+1 is staged into the other).
+constexpr int kMsStagedRows = 2;
 
 // A staged density value: rgb only (12 bytes, 4-byte aligned), so the
 // four taps of a sample cost 48 bytes of LDS reads, not 64.
@@ -836,24 +916,28 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
                                                              float4* __restrict__ S) {
-    extern __shared__ Rgb s_row[];              // [2][W + MUS + 1]: staged density rows
-    __shared__ float4 s_desc[kMsRowFloat4s];    // the row's entries of the table MR
-    const float4* s_tw = s_desc;                // T(r,mu,d_i)*dx*weight (rgb), d_i
-    const float4* s_geo = s_desc + kMsSamples;  // 1/r_i, wy, wz
-    const int4* s_off = reinterpret_cast<const int4*>(s_desc + 2 * kMsSamples);
+    extern __shared__ Rgb s_row[];              // [2][(NU + 1) * (MUS + 1)]: staged density rows
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
     const int y = bx, z = by;
     const int tid = threadIdx.x, nt = blockDim.x;
 
-    // Row geometry: the 51 samples' terms, written by the single-scattering
-    // pass (the same samples, single_scattering.comp:30-65).
-    const float4* desc = MR + (size_t(z) * MU + y) * kMsRowFloat4s;
-    for (int e = tid; e < kMsRowFloat4s; e += nt) s_desc[e] = desc[e];
+    // Row geometry: the 51 samples' terms, written by the geometry pass (the
+    // same samples, single_scattering.comp:30-65).  Every read of it is
+    // block-uniform, and the table is read-only here: read through the
+    // constant address space, the reads are scalar loads.
+    typedef FB_CONSTANT_AS const float4 CF4;
+    typedef FB_CONSTANT_AS const int4 CI4;
+    CF4* desc = (CF4*)(MR + (size_t(z) * MU + y) * kMsRowFloat4s);
+    auto d_tw = [&](int i) { return desc[i]; };  // T(r,mu,d_i)*dx*weight (rgb), d_i
+    auto d_geo = [&](int i) { return desc[kMsSamples + i]; };  // 1/r_i, slot load mask
+    auto d_off = [&](int i) { return ((CI4*)(desc + 2 * kMsSamples))[i]; };  // slot rows
+    auto d_wt = [&](int i) { return desc[3 * kMsSamples + i]; };  // slot weights
     const G r = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z)).r;
 
     // Per-texel constants: mu_s, nu, nu slice and its lerp weight.
     float tmu_s[TPT], tnu[TPT], tslice[TPT], tlerp[TPT];
+    int spos[TPT];  // the texel's entry in a staged row
     Sum<COMP> acc[TPT][3];
     const float rf = float(r);
 #pragma unroll
@@ -863,6 +947,7 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         tnu[q] = 0.0f;
         tslice[q] = 0.0f;
         tlerp[q] = 0.0f;
+        spos[q] = x + x / MUS;
         if (x < W) {
             TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
             tmu_s[q] = float(c.mu_s);
@@ -877,47 +962,74 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
 
     // GetScatteringTextureUvwzFromRMuMuSNu's mu_s coordinate (scattering.h:
     // 48-56) folded into the texel coordinate of slice `tslice`:
-    // x_tex = x_mu_s*(MUS-1) + tslice*MUS, with reciprocals hoisted.
+    // x_tex = x_mu_s*(MUS-1) + tslice*MUS, with reciprocals hoisted; in a
+    // staged row, slice k starts at entry k*(MUS + 1).
     const float b = P.bottom, b2 = P.bottom * P.bottom;
     const float H2 = P.top * P.top - P.bottom * P.bottom;
     const float d_min = P.top - P.bottom;
     const float inv_range = 1.0f / (sqrtf(H2) - d_min);
     const float invA = 1.0f / (-2.0f * P.mu_s_min * P.bottom * inv_range);
-    const float musm1 = float(MUS - 1), fMUS = float(MUS);
+    const int SS = MUS + 1;
+    const float musm1 = float(MUS - 1), fSS = float(SS);
 
-    // The (mu, r)-bilinear density row of sample i, for texel x.
-    struct Rows {
-        float4 r00, r01, r10, r11;
+    // The four density-row slots of this thread's texels x = tid + q*nt,
+    // carried from sample to sample (ms_row_slots): each sample loads only
+    // the rows it does not share with the previous one.
+    Rgb slot[TPT][4];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) slot[q][k] = Rgb{0.0f, 0.0f, 0.0f};
+    auto ld = [&](int o) {
+        const float4 v = ldg4(J + o);
+        return Rgb{v.x, v.y, v.z};
     };
-    auto fetch = [&](int i, int x) -> Rows {
-        const int4 o = s_off[i];
-        return {ldg4(J + o.x + x), ldg4(J + o.y + x), ldg4(J + o.z + x), ldg4(J + o.w + x)};
+    // Issue sample i's loads; they stay in flight until put() reads them.
+    auto advance = [&](int i) {
+        const int mask = __float_as_int(d_geo(i).y);
+        const int4 o = d_off(i);
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const int x = tid + q * nt;
+            if (x < W) {
+                if (mask & 1) slot[q][0] = ld(o.x + x);
+                if (mask & 2) slot[q][1] = ld(o.y + x);
+                if (mask & 4) slot[q][2] = ld(o.z + x);
+                if (mask & 8) slot[q][3] = ld(o.w + x);
+            }
+        }
     };
-    // Row layout: entries 0..W-1, then entry W = entry W-1 (the right edge
-    // of CLAMP_TO_EDGE), then MUS zero entries that only ever carry weight
-    // 0 (the nu slice above the last one, where the nu lerp weight is 0).
-    // The padding lets every tap index go unclamped.
-    const int WP = W + MUS + 1;
-    for (int e = tid; e < MUS; e += nt) {
-        s_row[W + 1 + e] = Rgb{0.0f, 0.0f, 0.0f};
-        s_row[WP + W + 1 + e] = Rgb{0.0f, 0.0f, 0.0f};
+    // Row layout: nu slice k (k = 0..NU-1) at entries k*SS .. k*SS+MUS-1,
+    // then one pad entry; then a zero slice NU.  Pads and the zero slice are
+    // read only with weight 0 (the mu_s tap after a slice's last entry has
+    // weight 0, and so has the nu slice above the last one), so every tap
+    // index goes unclamped.  The pad entry shifts consecutive slices by one
+    // LDS bank (taps of neighbouring slices at the same mu_s would otherwise
+    // share banks).
+    const int WP = (NU + 1) * SS;
+    for (int e = tid; e < NU + 1 + MUS; e += nt) {
+        const int k = e < NU ? e * SS + MUS : NU * SS + (e - NU);  // pads, then the zero slice
+#pragma unroll
+        for (int b = 0; b < kMsStagedRows; ++b) s_row[b * WP + k] = Rgb{0.0f, 0.0f, 0.0f};
     }
-    auto put = [&](int i, const Rows& q, Rgb* dst, int x) {
-        const float4 gq = s_geo[i];
-        float ax = fmaf(gq.y, q.r01.x - q.r00.x, q.r00.x),
-              ay = fmaf(gq.y, q.r01.y - q.r00.y, q.r00.y),
-              az = fmaf(gq.y, q.r01.z - q.r00.z, q.r00.z);
-        float bx = fmaf(gq.y, q.r11.x - q.r10.x, q.r10.x),
-              by = fmaf(gq.y, q.r11.y - q.r10.y, q.r10.y),
-              bz = fmaf(gq.y, q.r11.z - q.r10.z, q.r10.z);
-        const Rgb v{fmaf(gq.z, bx - ax, ax), fmaf(gq.z, by - ay, ay), fmaf(gq.z, bz - az, az)};
+    // The (mu, r)-bilinear density value of sample i at texel x (the slot
+    // weights are the bilinear weights), into the staged row dst.
+    auto put = [&](int i, const Rgb* t, Rgb* dst, int x) {
+        const float4 w = d_wt(i);
+        const Rgb v{fmaf(w.w, t[3].x, fmaf(w.z, t[2].x, fmaf(w.y, t[1].x, w.x * t[0].x))),
+                    fmaf(w.w, t[3].y, fmaf(w.z, t[2].y, fmaf(w.y, t[1].y, w.x * t[0].y))),
+                    fmaf(w.w, t[3].z, fmaf(w.z, t[2].z, fmaf(w.y, t[1].z, w.x * t[0].z)))};
         dst[x] = v;
-        if (x == W - 1) dst[W] = v;
+    };
+    auto put_all = [&](int i, Rgb* dst) {
+#pragma unroll
+        for (int q = 0; q < TPT; ++q)
+            if (tid + q * nt < W) put(i, slot[q], dst, spos[q]);
     };
     // Sample i of every texel of this thread, from the staged row `cur`.
     auto eval = [&](int i, const Rgb* __restrict__ cur) {
-        const float4 tw = s_tw[i];
-        const float inv_r_i = s_geo[i].x, d_i = tw.w;
+        const float4 tw = d_tw(i);
+        const float inv_r_i = d_geo(i).x, d_i = tw.w;
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             const int x = tid + q * nt;
@@ -931,15 +1043,15 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
                 // max(1 - a/A, 0) / (1 + a), kept <= 1 (a >= 0 up to rounding).
                 float xm = __builtin_amdgcn_fmed3f(
                     fmaf(-a, invA, 1.0f) * __builtin_amdgcn_rcpf(1.0f + a), 0.0f, 1.0f);
-                float xt = fmaf(xm, musm1, tslice[q] * fMUS);
+                float xt = fmaf(xm, musm1, tslice[q] * fSS);
                 float fl = floorf(xt);
                 float w = xt - fl;
                 int i0 = int(fl);
-                // 0 <= i0 <= W - 1 (0 <= xm <= 1 up to rounding, 0 <= tslice
-                // <= NU - 1): the taps i0, i0+1, i0+MUS, i0+MUS+1 stay inside
+                // 0 <= i0 - tslice*SS <= MUS - 1 (0 <= xm <= 1, 0 <= tslice
+                // <= NU - 1): the taps i0, i0+1, i0+SS, i0+SS+1 stay inside
                 // the padded row (see the layout above).
-                const Rgb p0 = cur[i0], p1 = cur[i0 + 1], p2 = cur[i0 + MUS],
-                          p3 = cur[i0 + MUS + 1];
+                const Rgb p0 = cur[i0], p1 = cur[i0 + 1], p2 = cur[i0 + SS],
+                          p3 = cur[i0 + SS + 1];
                 float l = tlerp[q];
                 float v0 = fmaf(w, p1.x - p0.x, p0.x), v1 = fmaf(w, p1.y - p0.y, p0.y),
                       v2 = fmaf(w, p1.z - p0.z, p0.z);
@@ -951,29 +1063,19 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
             }
         }
     };
-    for (int x = tid; x < W; x += nt) put(0, fetch(0, x), s_row, x);
+    // (Each thread stages exactly its own texels x = tid + q*nt: W <= TPT*nt.)
+    advance(0);
+    put_all(0, s_row);
     __syncthreads();
     for (int i = 0; i < kMsSamples; ++i) {
         const Rgb* cur = s_row + (i & 1) * WP;
         Rgb* nxt = s_row + ((i + 1) & 1) * WP;
         const bool next = i + 1 < kMsSamples;
-        if constexpr (TPT <= 2) {
-            // Issue the loads of sample i+1 first, so that their latency
-            // overlaps the evaluation of sample i.  (Each thread stages
-            // exactly its own texels x = tid + q*nt, since W <= TPT*nt.)
-            Rows pre[TPT];
-#pragma unroll
-            for (int q = 0; q < TPT; ++q)
-                if (next && tid + q * nt < W) pre[q] = fetch(i + 1, tid + q * nt);
-            eval(i, cur);
-#pragma unroll
-            for (int q = 0; q < TPT; ++q)
-                if (next && tid + q * nt < W) put(i + 1, pre[q], nxt, tid + q * nt);
-        } else {
-            if (next)
-                for (int x = tid; x < W; x += nt) put(i + 1, fetch(i + 1, x), nxt, x);
-            eval(i, cur);
-        }
+        // The loads of sample i+1 are issued first, so that their latency
+        // overlaps the evaluation of sample i.
+        if (next) advance(i + 1);
+        eval(i, cur);
+        if (next) put_all(i + 1, nxt);
         __syncthreads();
     }
 #pragma unroll
@@ -1002,7 +1104,16 @@ __device__ __forceinline__ void decode_tile(uint32_t t, int& atm, int& local) {
     atm = int(t >> 24);
     local = int(t & 0xffffffu);
 }
-
+// Batched kernels read the LUT addresses from memory.  Read through the
+// constant address space (the arrays are written by the host before the
+// launch and only read here), the loaded addresses are known to point to
+// global memory, so the compiler emits global rather than flat accesses
+// (flat ones also count against the LDS wait counter, serialising LDS and
+// memory waits).
+typedef FB_CONSTANT_AS const LutPtrs ConstLutPtrs;
+__device__ __forceinline__ LutPtrs lut_ptrs(const LutPtrs* L, int atm) {
+    return ((ConstLutPtrs*)L)[atm];
+}
 template <typename G, bool COMP, int LANES>
 __global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T,
                                                        float4* __restrict__ OD) {
@@ -1019,7 +1130,8 @@ __global__ __launch_bounds__(256) void kb_transmittance(const DevParams* __restr
         if (t != blockIdx.x) __syncthreads();
         int atm, local;
         decode_tile(tiles[t], atm, local);
-        transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, L[atm].T, L[atm].OD);
+        const LutPtrs l = lut_ptrs(L, atm);
+        transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, l.T, l.OD);
     }
 }
 
@@ -1041,7 +1153,8 @@ __global__ __launch_bounds__(256) void kb_direct_irradiance(const DevParams* __r
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const int gx = (P.i_mus + 63) / 64;
-        direct_irradiance_block<G>(P, local % gx, local / gx, 0, L[atm].T, L[atm].dE);
+        const LutPtrs l = lut_ptrs(L, atm);
+        direct_irradiance_block<G>(P, local % gx, local / gx, 0, l.T, l.dE);
     }
 }
 
@@ -1082,7 +1195,7 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
         int atm, local;
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
-        const LutPtrs& l = L[atm];
+        const LutPtrs l = lut_ptrs(L, atm);
         single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.SR, l.dR,
                                          l.dM, l.S);
     }
@@ -1129,7 +1242,7 @@ __global__ __launch_bounds__(64) void kb_geometry(const DevParams* __restrict__ 
     for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
         int atm, local;
         decode_tile(tiles[t], atm, local);
-        const LutPtrs& l = L[atm];
+        const LutPtrs l = lut_ptrs(L, atm);
         geometry_block<G>(Ps[atm], local, l.T, l.OD, l.DG, l.SR, l.MR);
     }
 }
@@ -1145,7 +1258,7 @@ __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams*
         int atm, local;
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
-        const LutPtrs& l = L[atm];
+        const LutPtrs l = lut_ptrs(L, atm);
         const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
         scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
                                             local / (gx * P.s_mus), l.DG, l.dR, l.dM, l.dMS, l.dE,
@@ -1170,13 +1283,13 @@ __global__ __launch_bounds__(64 * kIndWaves) void kb_indirect_irradiance(
         if (t != blockIdx.x) __syncthreads();
         int atm, local;
         decode_tile(tiles[t], atm, local);
-        const LutPtrs& l = L[atm];
+        const LutPtrs l = lut_ptrs(L, atm);
         indirect_irradiance_block<G, SINGLE>(Ps[atm], local, 0, 0, l.dR, l.dM, l.dMS, l.dE, l.E);
     }
 }
 
 template <typename G, bool COMP, int TPT>
-__global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
+__global__ __launch_bounds__(kMsMaxThreads) void k_multiple_scattering(DevParams P,
                                                              const float4* __restrict__ MR,
                                                              const float4* __restrict__ J,
                                                              float4* __restrict__ dMS,
@@ -1186,7 +1299,7 @@ __global__ __launch_bounds__(256) void k_multiple_scattering(DevParams P,
     multiple_scattering_block<G, COMP, TPT>(P, b % P.s_mu, b / P.s_mu, 0, MR, J, dMS, S);
 }
 template <typename G, bool COMP, int TPT>
-__global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
+__global__ __launch_bounds__(kMsMaxThreads) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
                                                               const LutPtrs* __restrict__ L,
                                                               const uint32_t* __restrict__ tiles, unsigned ntiles) {
     // Persistent when the grid is smaller than the tile list: the block
@@ -1197,7 +1310,7 @@ __global__ __launch_bounds__(256) void kb_multiple_scattering(const DevParams* _
         int atm, local;
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
-        const LutPtrs& l = L[atm];
+        const LutPtrs l = lut_ptrs(L, atm);
         multiple_scattering_block<G, COMP, TPT>(P, local % P.s_mu, local / P.s_mu, 0, l.MR, l.J,
                                                 l.dMS, l.S);
     }
@@ -1221,7 +1334,7 @@ __global__ void kb_clear_irradiance(const DevParams* __restrict__ Ps,
         decode_tile(tiles[t], atm, local);
         const size_t n = size_t(Ps[atm].i_mus) * Ps[atm].i_r;
         const size_t i = size_t(local) * blockDim.x + threadIdx.x;
-        if (i < n) L[atm].E[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < n) lut_ptrs(L, atm).E[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
@@ -1321,7 +1434,7 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, cons
     const unsigned remap = (tu.remap >> kPassMultiple) & 1u;
     if (W > threads * kMsTexelsPerThread) return hipErrorInvalidValue;
     dim3 grid(P.s_mu, P.s_r);
-    size_t lds = sizeof(Rgb) * 2 * size_t(W + P.s_mus + 1);
+    size_t lds = sizeof(Rgb) * kMsStagedRows * size_t(P.s_nu + 1) * (P.s_mus + 1);
     // Texels per thread: 1 up to 256 texels per row, else 2 or 4.  The
     // arithmetic of a texel does not depend on it.
     const int tpt = W <= threads ? 1 : (W <= 2 * threads ? 2 : 4);
@@ -1341,11 +1454,12 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, cons
 // (density n, the indirect pass of the order-n loop iteration, multiple n).
 unsigned default_threads(const DevParams& P, int pass, int order) {
     const unsigned W = unsigned(P.s_nu * P.s_mus);
-    const unsigned row = std::min(256u, (W + 63) / 64 * 64);
+    const unsigned row = (W + 63) / 64 * 64;
     switch (pass) {
         case kPassSingle:
-        case kPassMultiple:
-            return row;
+            return std::min(256u, row);
+        case kPassMultiple:  // one texel per thread
+            return std::min(unsigned(kMsMaxThreads), row);
         case kPassDensity:
             return 256;
         case kPassIndirect: {
@@ -1370,13 +1484,15 @@ unsigned tuned_threads(const DevParams& P, int pass, int order, const Tuning& tu
 }
 
 bool tuning_valid(const DevParams& P, int pass, int order, unsigned threads) {
-    if (threads == 0 || threads % 64 != 0 || threads > 256) return false;
+    if (threads == 0 || threads % 64 != 0) return false;
     const unsigned W = unsigned(P.s_nu * P.s_mus);
+    if (pass == kPassMultiple)  // above 256 threads, at most one wavefront more than the row
+        return threads <= unsigned(kMsMaxThreads) && (threads <= 256 || threads < W + 64) &&
+               W <= threads * unsigned(kMsTexelsPerThread);
+    if (threads > 256) return false;
     switch (pass) {
         case kPassSingle:
             return true;
-        case kPassMultiple:
-            return W <= threads * unsigned(kMsTexelsPerThread);
         case kPassDensity:
             return true;
         case kPassIndirect:
@@ -1413,7 +1529,7 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
         }
         case kPassMultiple:
             return {unsigned(P.s_mu) * unsigned(P.s_r), threads,
-                    sizeof(Rgb) * 2 * (size_t(P.s_nu) * P.s_mus + P.s_mus + 1)};
+                    sizeof(Rgb) * kMsStagedRows * size_t(P.s_nu + 1) * (P.s_mus + 1)};
     }
     return {0, 0, 0};
 }

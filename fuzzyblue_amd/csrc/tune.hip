@@ -93,7 +93,8 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
     }
 
     constexpr int kWarm = 2, kReps = 5;
-    const unsigned candidates[] = {64, 128, 256};
+    // (Only multiple scattering accepts more than 256 threads.)
+    const unsigned candidates[] = {64, 128, 256, 512, 1024};
     fbk::Tuning best = builder_tuning(b, *p);
     float best_ms[6] = {0, 0, 0, 0, 0, 0};
 
@@ -147,7 +148,6 @@ fb_status fb_builder_autotune(fb_builder* b, fb_stream stream, const fb_paramete
     float4 *T = a->T.p, *E = a->E.p, *S = a->S.p;
     float4 *dE = pa->dE.p, *dR = pa->dR.p, *dM = pa->dM.p, *dMS = pa->dMS.p, *J = pa->J.p,
            *MR = pa->MR.p;
-    const float4* OD = (p->flags & FB_FLAG_PRECISE) ? pa->OD.p : nullptr;
     const float4* DG = pa->DG.p;
     const float4* SR = pa->SR.p;
     // (The repeated launches rewrite the scratch set's LUTs, and the
@@ -249,7 +249,7 @@ fb_status fb_builder_set_tuning(fb_builder* b, const fb_parameters* p, const fb_
     tu.remap = t->xcd_remap;
     for (const Knob& k : knobs) {
         if (k.v == 0) continue;  // default
-        if (k.v > 256 || !fbk::tuning_valid(P, k.pass, k.order, k.v * k.unit)) {
+        if (k.v > 1024 || !fbk::tuning_valid(P, k.pass, k.order, k.v * k.unit)) {
             set_error("block size %u is not valid for pass %d at these sizes", k.v, k.pass);
             return FB_EINVAL;
         }

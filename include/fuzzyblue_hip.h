@@ -176,7 +176,10 @@ typedef struct fb_tuning {
     uint32_t density_threads;             /* the same for orders >= 3 */
     uint32_t indirect_order1_waves;       /* 1, 2 or 4 wavefronts per block */
     uint32_t indirect_waves;              /* the same for orders >= 2 */
-    uint32_t multiple_scattering_threads; /* 64, 128 or 256 per (r, mu) row */
+    uint32_t multiple_scattering_threads; /* 64 .. 1024 (a multiple of 64) per
+                                             (r, mu) row: at most 4 texels per
+                                             thread, and above 256 at most one
+                                             wavefront more than the row */
     uint32_t xcd_remap; /* bit p set (p = FB_PASS_SINGLE_SCATTERING,
                            FB_PASS_SCATTERING_DENSITY or
                            FB_PASS_MULTIPLE_SCATTERING): that pass gives each

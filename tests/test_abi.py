@@ -55,11 +55,11 @@ def test_footprint():
     assert res.value == (256 * 64 + 64 * 16 + ns) * 16
     # optical depths (transmittance-sized), irradiance delta, four
     # scattering-sized scratch LUTs and the multiple-scattering row table
-    # (3 float4 x 51 samples per (r, mu) row)
+    # (4 float4 x 51 samples per (r, mu) row)
     # the density geometry table (33 float4 per r, NU + 2 per mu_s) and the
     # single-scattering row table (5 float4 x 51 samples per row)
     rows = 32 * 128
-    assert scr.value == (256 * 64 + 64 * 16 + 4 * ns + rows * 3 * 51 + 32 * 33 + 32 * 10
+    assert scr.value == (256 * 64 + 64 * 16 + 4 * ns + rows * 4 * 51 + 32 * 33 + 32 * 10
                          + rows * 5 * 51) * 16
     assert tot == res.value + scr.value
 

@@ -62,7 +62,15 @@ def test_set_tuning_rejects_invalid_shapes():
         b.set_tuning(p, indirect_waves=8)
     with pytest.raises(FuzzyblueError):
         b.set_tuning(p, xcd_remap=1 << abi.FB_PASS_TRANSMITTANCE)
+    with pytest.raises(FuzzyblueError):
+        b.set_tuning(p, multiple_scattering_threads=2048)
+    with pytest.raises(FuzzyblueError):
+        b.set_tuning(p, single_scattering_threads=512)  # only multiple scattering goes past 256
     b.set_tuning(p, multiple_scattering_threads=256)
+    b.set_tuning(p, multiple_scattering_threads=1024)  # one texel per thread
+    small = Parameters().replace(**DUMP_SIZES)  # 64 texels per row
+    with pytest.raises(FuzzyblueError):
+        b.set_tuning(small, multiple_scattering_threads=512)
     b.close()
 
 
