@@ -295,7 +295,8 @@ __device__ __forceinline__ void row_geometry_block(const DevParams& P, const int
             make_float4(float(cos_h), float(d_min), float(d_max - d_min), tr.w);
         o[3 * kMsSamples + i] =
             make_float4(float(G(1) / r_d), float(G(1) / (d_max - d_min)),
-                        float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))), 0.0f);
+                        float(fmax(G(P.top) * G(P.top) - r_d * r_d, G(0))),
+                        float(G(1) / (G(2) * sin_h * G(P.sun_angular_radius))));
         o[4 * kMsSamples + i] = make_float4(__int_as_float(tr.i0 * P.t_mu),
                                             __int_as_float(tr.i1 * P.t_mu), float(dx), 0.0f);
     }
@@ -323,6 +324,20 @@ __device__ __forceinline__ void row_geometry_block(const DevParams& P, const int
     m[3 * kMsSamples + i] = sl.w;
 }
 
+// A raw buffer resource over `bytes` bytes at p (descriptor word 3 for
+// untyped 32-bit loads on gfx9-family parts); loads past the end return 0.
+// Offsets are 32-bit, so the tap addresses need no 64-bit arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), short(0), bytes, 0x00020000);
+}
+__device__ __forceinline__ float3 buffer_rgb(__amdgpu_buffer_rsrc_t r, int byte_offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, byte_offset, 0, 0);
+    return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
+__device__ __forceinline__ float3 lerp3(float3 a, float3 b, float w) {
+    return make_float3(fmaf(w, b.x - a.x, a.x), fmaf(w, b.y - a.y, a.y), fmaf(w, b.z - a.z, a.z));
+}
+
 template <typename G, bool COMP>
 __device__ __forceinline__ void single_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                            const float4* __restrict__ T,
@@ -330,33 +345,36 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
                                                            float4* __restrict__ dR,
                                                            float4* __restrict__ dM,
                                                            float4* __restrict__ S) {
-    __shared__ float4 s_row[kSsRowFloat4s];
-    const float4* s_tv = s_row;                    // T(r,mu,d_i) * weight (rgb), d_i
-    const float4* s_geo = s_row + kMsSamples;      // r_d, rho_rayleigh, rho_mie, sin_theta_h
-    const float4* s_lut = s_row + 2 * kMsSamples;  // cos_theta_h, d_min, d_max - d_min, T-row weight
-    const float4* s_inv = s_row + 3 * kMsSamples;  // 1/r_d, 1/(d_max - d_min), top^2 - r_d^2
-    const float4* s_trw = s_row + 4 * kMsSamples;  // T-row offsets (int bits), dx
+    // The row table (read-only here, every read block-uniform: scalar loads
+    // through the constant address space).
+    typedef FB_CONSTANT_AS const float4 CF4;
+    CF4* row = (CF4*)(SR + (size_t(by) * P.s_mu + bx) * kSsRowFloat4s);
+    CF4* s_tv = row;                    // T(r,mu,d_i) * weight (rgb), d_i
+    CF4* s_geo = row + kMsSamples;      // r_d, rho_rayleigh, rho_mie, sin_theta_h
+    CF4* s_lut = row + 2 * kMsSamples;  // cos_theta_h, d_min, d_max - d_min, T-row weight
+    CF4* s_inv = row + 3 * kMsSamples;  // 1/r_d, 1/(d_max - d_min), top^2 - r_d^2, 1/(2 sin_theta_h alpha)
+    CF4* s_trw = row + 4 * kMsSamples;  // T-row offsets (int bits), dx
     constexpr bool kPrecise = sizeof(G) == sizeof(double);
     const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
     const int W = NU * MUS;
     const int y = bx, z = by;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const float4* row = SR + (size_t(z) * MU + y) * kSsRowFloat4s;
-    for (int e = tid; e < kSsRowFloat4s; e += nt) s_row[e] = row[e];
     const G r = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z)).r;
-    __syncthreads();
     const float rf = float(r);
-    const float alpha = P.sun_angular_radius;
     const float top2 = P.top * P.top;
-    const float tmax = float(P.t_mu - 1);
+    const int TMU = P.t_mu;
+    const float tmax = float(TMU - 1);
     for (int x = tid; x < W; x += nt) {
         TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
         const float mu_s = float(c.mu_s), nu = float(c.nu);
         Sum<COMP> rs[3], ms[3];
         for (int i = 0; i < kMsSamples; ++i) {
-            const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i];
+            const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i], iv = s_inv[i];
             const float4 tr4 = s_trw[i];
-            const int2 rows = make_int2(__float_as_int(tr4.x), __float_as_int(tr4.y));
+            // The sample's two T rows (r taps) as one buffer: row 0 at byte 0,
+            // row 1 at `drow` (0 when both taps are the same row).
+            const int row0 = __float_as_int(tr4.x), drow = (__float_as_int(tr4.y) - row0) * 16;
+            const __amdgpu_buffer_rsrc_t tb = buffer_rsrc(T + row0, drow + TMU * 16);
             const float r_d = gq.x, d_i = tv.w;
             Taps tx;
             float mu_s_d;
@@ -365,7 +383,6 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
                 // for mu_s_d > 0 the root is (top^2 - r_d^2) / (r_d mu_s_d +
                 // sqrt(disc)), with top^2 - r_d^2 from the FP64 set-up.
                 // Reciprocals of the per-sample divisors come from FP64 too.
-                const float4 iv = s_inv[i];
                 mu_s_d = __builtin_amdgcn_fmed3f(fmaf(rf, mu_s, d_i * nu) * iv.x, -1.0f, 1.0f);
                 const float rm = r_d * mu_s_d;
                 const float sq = __builtin_amdgcn_sqrtf(fmaf(rm, rm, iv.z));
@@ -374,7 +391,7 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
                 const float t = __builtin_amdgcn_fmed3f((d - lq.y) * iv.y * tmax, 0.0f, tmax);
                 const float fl = floorf(t);
                 tx.i0 = int(fl);
-                tx.i1 = min(tx.i0 + 1, P.t_mu - 1);
+                tx.i1 = min(tx.i0 + 1, TMU - 1);
                 tx.w = t - fl;
             } else {
                 mu_s_d = clamp_cosine<float>((rf * mu_s + d_i * nu) / r_d);
@@ -382,12 +399,16 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
                 float disc = r_d * r_d * (mu_s_d * mu_s_d - 1.0f) + top2;
                 float d = clamp_distance<float>(-r_d * mu_s_d + safe_sqrt<float>(disc));
                 float x_mu = (d - lq.y) / lq.z;
-                tx = linear_taps<float>(tex_from_unit<float>(x_mu, P.t_mu), P.t_mu);
+                tx = linear_taps<float>(tex_from_unit<float>(x_mu, TMU), TMU);
             }
-            float4 ts = lerp4(lerp4(ldg4(T + rows.x + tx.i0), ldg4(T + rows.x + tx.i1), tx.w),
-                              lerp4(ldg4(T + rows.y + tx.i0), ldg4(T + rows.y + tx.i1), tx.w),
-                              lq.w);
-            float sm = smoothstep_g<float>(-gq.w * alpha, gq.w * alpha, mu_s_d - lq.x);
+            const int o0 = tx.i0 * 16, o1 = tx.i1 * 16;
+            const float3 ts = lerp3(lerp3(buffer_rgb(tb, o0), buffer_rgb(tb, o1), tx.w),
+                                    lerp3(buffer_rgb(tb, drow + o0), buffer_rgb(tb, drow + o1), tx.w),
+                                    lq.w);
+            // smoothstep(-sin_h alpha, sin_h alpha, mu_s_d - cos_h) with the
+            // divisor's reciprocal from the table.
+            const float u = __builtin_amdgcn_fmed3f(fmaf(mu_s_d - lq.x, iv.w, 0.5f), 0.0f, 1.0f);
+            const float sm = u * u * (3.0f - 2.0f * u);
             float t0 = tv.x * (ts.x * sm), t1 = tv.y * (ts.y * sm), t2 = tv.z * (ts.z * sm);
             rs[0].add(t0 * gq.y);
             rs[1].add(t1 * gq.y);
