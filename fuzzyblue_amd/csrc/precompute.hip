@@ -123,7 +123,7 @@ struct Sum {
 // reference recomputes r_i per profile with identical inputs), then the
 // partial sums are combined with lane shuffles.  The FP32 mode uses one
 // lane, i.e. the reference's sequential summation order, so its rounding
-// matches the reference's; the precision mode splits the sum over 8 lanes
+// matches the reference's; the precision mode splits the sum over 16 lanes
 // with compensated partial sums.
 template <typename G, bool COMP, int LANES>
 __device__ __forceinline__ void transmittance_block(const DevParams& P, const int bx, const int by, const int bz, float4* __restrict__ T, float4* __restrict__ OD) {
@@ -142,6 +142,9 @@ __device__ __forceinline__ void transmittance_block(const DevParams& P, const in
     Sum<COMP> lr, lm, lo;
     for (int k = sub; k <= SAMPLE_COUNT; k += LANES) {
         G d_i = G(k) * dx;
+        // (r_i - bottom exactly as the oracle forms it: the density profile
+        // switches layer at altitude 0, so a sample at the horizon ray's
+        // tangent point must round the same way.)
         G r_i = sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r);
         float alt = float(r_i - G(P.bottom));
         float w = (k == 0 || k == SAMPLE_COUNT) ? 0.5f : 1.0f;
@@ -1365,7 +1368,7 @@ static inline unsigned cdiv(size_t a, size_t b) { return unsigned((a + b - 1) / 
 hipError_t launch_transmittance(const DevParams& P, float4* T, float4* OD, hipStream_t s) {
     size_t texels = size_t(P.t_mu) * P.t_r;
     if (P.flags & 1)
-        k_transmittance<double, true, 8><<<cdiv(texels * 8, 256), 256, 0, s>>>(P, T, OD);
+        k_transmittance<double, true, 16><<<cdiv(texels * 16, 256), 256, 0, s>>>(P, T, OD);
     else
         k_transmittance<float, false, 1><<<cdiv(texels, 64), 64, 0, s>>>(P, T, nullptr);
     return hipGetLastError();
@@ -1529,7 +1532,7 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
     const bool precise = P.flags & 1;
     switch (pass) {
         case kPassTransmittance: {
-            const size_t lanes = precise ? 8 : 1;
+            const size_t lanes = precise ? 16 : 1;
             return {cdiv(size_t(P.t_mu) * P.t_r * lanes, 256), 256, 0};
         }
         case kPassDirect:
@@ -1565,7 +1568,7 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
     switch (pass) {
         case kPassTransmittance:
             if (precise)
-                kb_transmittance<double, true, 8><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+                kb_transmittance<double, true, 16><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             else
                 kb_transmittance<float, false, 1><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
@@ -1632,7 +1635,7 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
     const void* fn = nullptr;
     switch (pass) {
         case kPassTransmittance:
-            fn = precise ? (const void*)kb_transmittance<double, true, 8>
+            fn = precise ? (const void*)kb_transmittance<double, true, 16>
                          : (const void*)kb_transmittance<float, false, 1>;
             break;
         case kPassDirect:
