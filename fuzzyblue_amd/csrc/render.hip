@@ -105,6 +105,76 @@ __device__ void sky_radiance_to_point(const DevParams& P, const float4* __restri
     rad = make_float3(sc.x * pr + mie.x * pm, sc.y * pr + mie.y * pm, sc.z * pr + mie.z * pm);
 }
 
+// The LUTs at one (r, mu_s).  For a camera inside the atmosphere every ray
+// of a frame shares the camera's r and, for one sun, mu_s: the r and mu_s
+// interpolation of the scattering lookup (scattering.h:139-155) and the r
+// interpolation of the transmittance lookup (transmittance.h:26-33) are the
+// same for all of its pixels.  A block evaluates them once into LDS --
+// Q[k][y] for nu slice k = 0..NU (slice NU is the clamped edge the lookup
+// of the last slice's upper neighbour reads) and mu texel y, then Trow[m]
+// for each transmittance mu texel -- and each pixel interpolates in mu and
+// nu only: 4 + 2 LDS reads instead of 16 + 4 texel fetches, the same value
+// up to the order of rounding.
+__host__ __device__ inline size_t sky_slice_bytes(const DevParams& P) {
+    return sizeof(float4) * (size_t(P.s_nu + 1) * P.s_mu + P.t_mu);
+}
+constexpr size_t kSkySliceMaxBytes = 65536;
+
+__device__ void build_sky_slice(const DevParams& P, const float4* __restrict__ T,
+                                const float4* __restrict__ S, float r, float mu_s,
+                                float4* __restrict__ lds) {
+    const int NU = P.s_nu, MU = P.s_mu, W = NU * P.s_mus, TMU = P.t_mu;
+    const Taps tz = linear_taps<float>(scattering_u_r<float>(P, r), P.s_r);
+    const float u_mu_s = scattering_u_mu_s<float>(P, mu_s);
+    for (int e = threadIdx.x; e < (NU + 1) * MU; e += blockDim.x) {
+        const int k = e / MU, y = e - k * MU;
+        const Taps tx = linear_taps<float>((float(k) + u_mu_s) / float(NU), W);
+        const float4* p0 = S + (size_t(tz.i0) * MU + y) * W;
+        const float4* p1 = S + (size_t(tz.i1) * MU + y) * W;
+        lds[e] = lerp4(lerp4(ldg4(p0 + tx.i0), ldg4(p0 + tx.i1), tx.w),
+                       lerp4(ldg4(p1 + tx.i0), ldg4(p1 + tx.i1), tx.w), tz.w);
+    }
+    float u, v;
+    transmittance_uv<float>(P, r, 1.0f, u, v);  // v depends on r only
+    const Taps tv = linear_taps<float>(v, P.t_r);
+    float4* trow = lds + (NU + 1) * MU;
+    for (int m = threadIdx.x; m < TMU; m += blockDim.x)
+        trow[m] = lerp4(ldg4(T + tv.i0 * TMU + m), ldg4(T + tv.i1 * TMU + m), tv.w);
+}
+
+// sky_radiance for a camera inside the atmosphere (so no move to the top
+// boundary), from the block's slice at the camera's r and mu_s.
+__device__ void sky_radiance_slice(const DevParams& P, const float4* __restrict__ lds, float r,
+                                   float3 camera, float3 view, float3 sun, float3& rad,
+                                   float3& trans) {
+    const int NU = P.s_nu, MU = P.s_mu;
+    const float mu = dot3(camera, view) / r;
+    const float nu = dot3(view, sun);
+    const bool ground = ray_intersects_ground<float>(P, r, mu);
+    if (ground) {
+        trans = make_float3(0.0f, 0.0f, 0.0f);
+    } else {
+        float u, v;
+        transmittance_uv<float>(P, r, mu, u, v);
+        const Taps tu = linear_taps<float>(u, P.t_mu);
+        const float4* trow = lds + (NU + 1) * MU;
+        const float4 t = lerp4(trow[tu.i0], trow[tu.i1], tu.w);
+        trans = make_float3(t.x, t.y, t.z);
+    }
+    const Taps ty = linear_taps<float>(scattering_u_mu<float>(P, r, mu, ground), MU);
+    const float tex_coord_x = (nu + 1.0f) / 2.0f * float(NU - 1);
+    const float tex_x = floorf(tex_coord_x);
+    const float l = tex_coord_x - tex_x;
+    const int k0 = int(tex_x);
+    const float4 a0 = lerp4(lds[k0 * MU + ty.i0], lds[k0 * MU + ty.i1], ty.w);
+    const float4 a1 = lerp4(lds[(k0 + 1) * MU + ty.i0], lds[(k0 + 1) * MU + ty.i1], ty.w);
+    const float4 c = make_float4(a0.x * (1.0f - l) + a1.x * l, a0.y * (1.0f - l) + a1.y * l,
+                                 a0.z * (1.0f - l) + a1.z * l, a0.w * (1.0f - l) + a1.w * l);
+    const float3 mie = extrapolated_single_mie(P, c);
+    const float pr = phase_rayleigh(nu), pm = phase_mie(P.g, nu);
+    rad = make_float3(c.x * pr + mie.x * pm, c.y * pr + mie.y * pm, c.z * pr + mie.z * pm);
+}
+
 struct DrawArgs {
     float ivp[4][4];  // [column][row]
     float cam[3];
@@ -206,27 +276,54 @@ __host__ __device__ inline void ray_direction(uint64_t seed, uint32_t frame, uin
     out[2] = 1.0f;
 }
 
+// Pixels per thread of the frame kernels (a block shares one slice).
+constexpr int kFramePixelsPerThread = 16;
+
+// Pixels [pix0, pix0 + 256*kFramePixelsPerThread) of one synthetic frame
+// (camera at altitude alt on the z axis, sun at zenith angle zs in the x-z
+// plane).  `slice`: the block's LDS holds room for the frame's LUT slice.
+__device__ void render_frame_block(const DevParams& P, const float4* __restrict__ T,
+                                   const float4* __restrict__ S, float alt, float zs,
+                                   uint32_t frame, uint32_t pixels, uint64_t seed, uint32_t pix0,
+                                   bool slice, float4* __restrict__ rad_out,
+                                   float4* __restrict__ trans_out) {
+    extern __shared__ float4 s_slice[];
+    const float3 cam = make_float3(0.0f, 0.0f, P.bottom + alt);
+    const float3 sun = make_float3(sinf(zs), 0.0f, cosf(zs));
+    const float r = sqrtf(dot3(cam, cam));
+    const bool use_slice = slice && r <= P.top;  // block-uniform
+    if (use_slice) {
+        build_sky_slice(P, T, S, r, dot3(cam, sun) / r, s_slice);
+        __syncthreads();
+    }
+    for (int j = 0; j < kFramePixelsPerThread; ++j) {
+        const uint32_t p = pix0 + uint32_t(j) * blockDim.x + threadIdx.x;
+        if (p >= pixels) break;
+        float d[3];
+        ray_direction(seed, frame, p, d);
+        const float3 view = make_float3(d[0], d[1], d[2]);
+        float3 rad, tr;
+        if (use_slice)
+            sky_radiance_slice(P, s_slice, r, cam, view, sun, rad, tr);
+        else
+            sky_radiance(P, T, S, cam, view, sun, rad, tr);
+        const size_t o = size_t(frame) * pixels + p;  // frame-major output
+        rad_out[o] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+        if (trans_out) trans_out[o] = make_float4(tr.x, tr.y, tr.z, 1.0f);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_render_frames(DevParams P, const float4* __restrict__ T,
                                                        const float4* __restrict__ S,
                                                        const float* __restrict__ altitudes,
                                                        const float* __restrict__ zeniths,
                                                        uint32_t pixels, uint64_t seed,
-                                                       uint32_t frame0,
-                                                       float4* __restrict__ rad_out,
+                                                       int slice, float4* __restrict__ rad_out,
                                                        float4* __restrict__ trans_out) {
-    uint32_t f = blockIdx.y;
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pixels) return;
-    float alt = altitudes[f], zs = zeniths[f];
-    float3 cam = make_float3(0.0f, 0.0f, P.bottom + alt);
-    float3 sun = make_float3(sinf(zs), 0.0f, cosf(zs));
-    float d[3];
-    ray_direction(seed, frame0 + f, p, d);
-    float3 rad, tr;
-    sky_radiance(P, T, S, cam, make_float3(d[0], d[1], d[2]), sun, rad, tr);
-    size_t o = size_t(f) * pixels + p;
-    rad_out[o] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-    if (trans_out) trans_out[o] = make_float4(tr.x, tr.y, tr.z, 1.0f);
+    const uint32_t f = blockIdx.y;
+    render_frame_block(P, T, S, altitudes[f], zeniths[f], f, pixels, seed,
+                       blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0, rad_out,
+                       trans_out);
 }
 
 // render_lighting.h:10-28
@@ -257,23 +354,29 @@ __global__ __launch_bounds__(256) void k_sun_sky_irradiance(
 __global__ __launch_bounds__(256) void k_render_frame1(DevParams P, const float4* __restrict__ T,
                                                        const float4* __restrict__ S, float alt,
                                                        float zs, uint32_t pixels, uint64_t seed,
-                                                       float4* __restrict__ rad_out) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pixels) return;
-    float3 cam = make_float3(0.0f, 0.0f, P.bottom + alt);
-    float3 sun = make_float3(sinf(zs), 0.0f, cosf(zs));
-    float d[3];
-    ray_direction(seed, 0u, p, d);
-    float3 rad, tr;
-    sky_radiance(P, T, S, cam, make_float3(d[0], d[1], d[2]), sun, rad, tr);
-    rad_out[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+                                                       int slice, float4* __restrict__ rad_out) {
+    render_frame_block(P, T, S, alt, zs, 0u, pixels, seed,
+                       blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0, rad_out,
+                       nullptr);
+}
+
+// Blocks and dynamic LDS of a frame launch.
+static unsigned frame_blocks(uint32_t pixels) {
+    const uint32_t per = 256u * kFramePixelsPerThread;
+    return (pixels + per - 1) / per;
+}
+static size_t frame_lds(const DevParams& P) {
+    const size_t b = sky_slice_bytes(P);
+    return b <= kSkySliceMaxBytes ? b : 0;
 }
 
 hipError_t launch_render_frame(const DevParams& P, const float4* T, const float4* S, float alt,
                                float zs, uint32_t pixels, uint64_t seed, float4* rad,
                                hipStream_t s) {
     if (pixels == 0) return hipSuccess;
-    k_render_frame1<<<(pixels + 255) / 256, 256, 0, s>>>(P, T, S, alt, zs, pixels, seed, rad);
+    const size_t lds = frame_lds(P);
+    k_render_frame1<<<frame_blocks(pixels), 256, lds, s>>>(P, T, S, alt, zs, pixels, seed,
+                                                           lds ? 1 : 0, rad);
     return hipGetLastError();
 }
 
@@ -367,11 +470,12 @@ fb_status fb_render_sky_frames(fb_builder* b, fb_stream stream, const fb_atmosph
     fb_atmosphere* a = const_cast<fb_atmosphere*>(ac);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if ((st = begin_use(a, s)) != FB_OK) return st;
-    dim3 grid((pixels + 255) / 256, n_frames);
-    fbk::k_render_frames<<<grid, 256, 0, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
-                                               sun_zenith_rad, pixels, seed, 0u,
-                                               reinterpret_cast<float4*>(radiance),
-                                               reinterpret_cast<float4*>(transmittance));
+    dim3 grid(fbk::frame_blocks(pixels), n_frames);
+    const size_t lds = fbk::frame_lds(a->dev);
+    fbk::k_render_frames<<<grid, 256, lds, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
+                                                 sun_zenith_rad, pixels, seed, lds ? 1 : 0,
+                                                 reinterpret_cast<float4*>(radiance),
+                                                 reinterpret_cast<float4*>(transmittance));
     return end_use(a, s);
 }
 
