@@ -510,6 +510,22 @@ __device__ __forceinline__ ProfEntry make_entry(float4 v, float4 w) {
     return e;
 }
 
+// Order 2 (single scattering as incident radiance): the Rayleigh and Mie
+// profile entries of one slot side by side, so that one lookup reads both
+// with three b128 LDS reads instead of two b128 + b64 pairs.
+struct PairEntry {
+    float4 a;  // R: v.r, v.g, v.b, d.r
+    float4 b;  // R: d.g, d.b; M: v.r, v.g
+    float4 c;  // M: v.b, d.r, d.g, d.b
+};
+__device__ __forceinline__ PairEntry make_pair_entry(float4 rv, float4 rw, float4 mv, float4 mw) {
+    PairEntry e;
+    e.a = make_float4(rv.x, rv.y, rv.z, rw.x - rv.x);
+    e.b = make_float4(rw.y - rv.y, rw.z - rv.z, mv.x, mv.y);
+    e.c = make_float4(mv.z, mw.x - mv.x, mw.y - mv.y, mw.z - mv.z);
+    return e;
+}
+
 // Per-texel constants of the density quadrature.
 struct DenTexel {
     float omx, omz;          // omega = (omx, 0, omz)
@@ -522,8 +538,8 @@ struct DenTexel {
 // Per-(texel, theta) constants.
 struct DenRow {
     float C, omzc;           // cos(theta)*omega_s.z, cos(theta)*omega.z
-    const ProfEntry* pr;     // nu profile (order n-1 >= 2: multiple; 1: rayleigh)
-    const ProfEntry* pm;     // nu profile of single Mie (order 1)
+    const ProfEntry* pr;     // nu profile of multiple scattering (order n-1 >= 2)
+    const PairEntry* pp;     // nu profiles of single Rayleigh and Mie (order 1)
     const float4* dir;       // c_den_dir row: (cos phi sin t, sin phi sin t, cos t, dw)
     float3 gv;               // T_g * albedo / pi (ground rows)
     float Ag, Bg, Cg;        // ground normal . omega_s = cx*Ag + sy*Bg + Cg
@@ -545,6 +561,17 @@ __device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof
     float2 b = e->b;
     return make_float3(fmaf(a.w, f, a.x), fmaf(b.x, f, a.y), fmaf(b.y, f, a.z));
 }
+// The same for a Rayleigh / Mie pair of profiles.
+__device__ __forceinline__ void pair_lerp_i(const PairEntry* __restrict__ prof, float t,
+                                            float kmax, float3& R, float3& M) {
+    t = __builtin_amdgcn_fmed3f(t, 0.0f, kmax);
+    const int k = int(t);
+    const float f = __builtin_amdgcn_fractf(t);
+    const PairEntry* e = prof + (k + 1);
+    const float4 a = e->a, b = e->b, c = e->c;
+    R = make_float3(fmaf(a.w, f, a.x), fmaf(b.x, f, a.y), fmaf(b.y, f, a.z));
+    M = make_float3(fmaf(c.y, f, b.z), fmaf(c.z, f, b.w), fmaf(c.w, f, c.x));
+}
 
 // The 16 direction pairs (phi_m, phi_{31-m}) of one theta row.  Both members
 // share cos(phi), hence nu2 = omega.omega_i and the phase functions; their
@@ -565,12 +592,12 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
         const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
         const float nua = base + sb, nub = base - sb;
         const float ta = fmaf(nua, dt.hnu, dt.hnu), tb = fmaf(nub, dt.hnu, dt.hnu);
-        float3 La = prof_lerp_i(row.pr, ta, dt.kmax);
-        float3 Lb = prof_lerp_i(row.pr, tb, dt.kmax);
+        float3 La, Lb;
         if (SINGLE) {
             // Profiles are pre-scaled by kR (rayleigh) and kM (mie).
-            const float3 Ma = prof_lerp_i(row.pm, ta, dt.kmax);
-            const float3 Mb = prof_lerp_i(row.pm, tb, dt.kmax);
+            float3 Ma, Mb;
+            pair_lerp_i(row.pp, ta, dt.kmax, La, Ma);
+            pair_lerp_i(row.pp, tb, dt.kmax, Lb, Mb);
             const float qa = fmaf(nua, nua, 1.0f), qb = fmaf(nub, nub, 1.0f);
             const float wa = rsqrt_cubed(fmaf(dt.g2, nua, dt.g1));
             const float wb = rsqrt_cubed(fmaf(dt.g2, nub, dt.g1));
@@ -578,6 +605,9 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
                              qa * fmaf(Ma.z, wa, La.z));
             Lb = make_float3(qb * fmaf(Mb.x, wb, Lb.x), qb * fmaf(Mb.y, wb, Lb.y),
                              qb * fmaf(Mb.z, wb, Lb.z));
+        } else {
+            La = prof_lerp_i(row.pr, ta, dt.kmax);
+            Lb = prof_lerp_i(row.pr, tb, dt.kmax);
         }
         if (GROUND) {
             const float gb = fmaf(cx, row.Ag, row.Cg), gs = sy * row.Bg;
@@ -734,13 +764,21 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     const float kRs = SINGLE ? 3.0f / (16.0f * kPi) : 1.0f;
     const float kMs =
         SINGLE ? 3.0f / (8.0f * kPi) * (1.0f - P.g * P.g) / (2.0f + P.g * P.g) : 1.0f;
-    for (int e = tid; e < NTEX * kDenTheta * nent; e += blockDim.x) {
+    // (Order 2: PairEntry [16][nent] in the same space, 48 of its 64 bytes
+    // per (row, entry).)
+    PairEntry* s_pair = reinterpret_cast<PairEntry*>(s_prof);
+    for (int e = tid; e < kDenTheta * nent; e += blockDim.x) {
         int row = e / nent, k = e % nent;  // entry k holds slot k-1
-        float sc = (SINGLE && row >= kDenTheta) ? kMs : kRs;
-        float4 v = s_raw[row * nslots + k], w = s_raw[row * nslots + k + 1];
-        v = make_float4(v.x * sc, v.y * sc, v.z * sc, 0.0f);
-        w = make_float4(w.x * sc, w.y * sc, w.z * sc, 0.0f);
-        s_prof[e] = make_entry(v, w);
+        auto scaled = [&](int rr, int kk, float sc) {
+            const float4 v = s_raw[rr * nslots + kk];
+            return make_float4(v.x * sc, v.y * sc, v.z * sc, 0.0f);
+        };
+        if (SINGLE)
+            s_pair[e] = make_pair_entry(scaled(row, k, kRs), scaled(row, k + 1, kRs),
+                                        scaled(kDenTheta + row, k, kMs),
+                                        scaled(kDenTheta + row, k + 1, kMs));
+        else
+            s_prof[e] = make_entry(scaled(row, k, kRs), scaled(row, k + 1, kRs));
     }
     for (int e = tid; e <= NE; e += blockDim.x) {
         int i0 = min(max(e - 1, 0), NE - 1), i1 = min(e, NE - 1);
@@ -795,7 +833,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         row.C = ct * dt.osz;
         row.omzc = ct * dt.omz;
         row.pr = s_prof + l * nent;
-        row.pm = s_prof + (kDenTheta + l) * nent;
+        row.pp = s_pair + l * nent;
         row.dir = c_den_dir + l * kDenPhi;
         if ((gmask >> l) & 1) {
             const float4 gv = s_gv[l];
