@@ -128,6 +128,32 @@ def test_frames_match_rays(builder, atm):
         assert rel_err(g[f], o_rad, 1e-6).max() < 1e-4
 
 
+def test_frames_slice_and_general_paths(builder, atm):
+    """Frame kernels evaluate the LUTs' r / mu_s interpolation once per frame
+    when the camera is inside the atmosphere; a camera above the top takes
+    the per-ray path.  Both agree with the oracle on the same hashed rays,
+    over more pixels than one block handles (4096)."""
+    a, T, S, _ = atm
+    px = 9000
+    alt = np.array([0.0, 59.9, 75.0], np.float32)  # bottom, just below top, above
+    zen = np.radians(np.array([30.0, 95.0, 60.0])).astype(np.float32)
+    r = Renderer(builder)
+    d_alt, d_zen = DeviceBuffer.from_array(alt), DeviceBuffer.from_array(zen)
+    rad = DeviceBuffer(len(alt) * px * 16)
+    tr = DeviceBuffer(len(alt) * px * 16)
+    r.frames(None, a, d_alt, d_zen, len(alt), px, 11, rad, tr)
+    synchronize()
+    g = rad.to_array(np.float32, (len(alt), px, 4))
+    gt = tr.to_array(np.float32, (len(alt), px, 4))
+    for f in range(len(alt)):
+        views = np.stack([ray_direction(11, f, i) for i in range(px)])
+        cams = np.array([[0.0, 0.0, np.float32(6360.0) + alt[f]]], np.float32)
+        sun = np.array([[math.sin(zen[f]), 0.0, math.cos(zen[f])]], np.float32)
+        o_rad, o_tr = Oracle("f32").sky_rays(P, T, S, cams, views, sun)
+        assert rel_err(g[f], o_rad, 1e-6).max() < 1e-4, f
+        assert rel_err(gt[f], o_tr, 1e-6).max() < 1e-4, f
+
+
 def test_draw_matches_fragment_shader(builder, atm):
     """render_sky.frag:24-35 over a frame with a depth buffer; composite
     mode reproduces the dual-source blend dst = color + dst * T."""
