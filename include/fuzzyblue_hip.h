@@ -283,6 +283,9 @@ typedef struct fb_lut_view {
     size_t bytes;
 } fb_lut_view;
 
+/* Any LUT of a pending atmosphere, scratch included (for inspection once
+ * its work completed).  FB_LUT_DELTA_IRRADIANCE is the last bounce's
+ * delta irradiance (the direct irradiance for order 1). */
 fb_status fb_pending_lut(const fb_pending_atmosphere *pa, fb_lut which,
                          fb_lut_view *out);
 
