@@ -1042,9 +1042,18 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
     for (int q = 0; q < TPT; ++q)
 #pragma unroll
         for (int k = 0; k < 4; ++k) slot[q][k] = Rgb{0.0f, 0.0f, 0.0f};
-    auto ld = [&](int o) {
-        const float4 v = ldg4(J + o);
-        return Rgb{v.x, v.y, v.z};
+    // Row loads as buffer loads: the row start (block-uniform) goes in the
+    // scalar offset and the texel's byte offset in a loop-invariant vector
+    // offset, so no address arithmetic writes a register between the loads
+    // and each load lands directly in its slot.  Loads past the LUT's end
+    // (texels x >= W of the last row) return 0 and are never used.
+    const __amdgpu_buffer_rsrc_t jb = buffer_rsrc(J, int(size_t(P.s_r) * MU * W * 16));
+    int xoff[TPT];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) xoff[q] = (tid + q * nt) * 16;
+    auto ld = [&](int o, int q) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(jb, xoff[q], o * 16, 0);
+        return Rgb{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
     };
     // Issue sample i's loads; they stay in flight until put() reads them.
     auto advance = [&](int i) {
@@ -1052,13 +1061,10 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         const int4 o = d_off(i);
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
-            const int x = tid + q * nt;
-            if (x < W) {
-                if (mask & 1) slot[q][0] = ld(o.x + x);
-                if (mask & 2) slot[q][1] = ld(o.y + x);
-                if (mask & 4) slot[q][2] = ld(o.z + x);
-                if (mask & 8) slot[q][3] = ld(o.w + x);
-            }
+            if (mask & 1) slot[q][0] = ld(o.x, q);
+            if (mask & 2) slot[q][1] = ld(o.y, q);
+            if (mask & 4) slot[q][2] = ld(o.z, q);
+            if (mask & 8) slot[q][3] = ld(o.w, q);
         }
     };
     // Row layout: nu slice k (k = 0..NU-1) at entries k*SS .. k*SS+MUS-1,
