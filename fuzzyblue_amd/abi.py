@@ -33,6 +33,7 @@ STATUS_NAMES = {
 }
 
 FB_FLAG_PRECISE = 0x1
+FB_FLAG_F16_STORAGE = 0x2
 
 # fb_lut
 FB_LUT_TRANSMITTANCE = 0

@@ -272,10 +272,11 @@ class PendingAtmosphere:
         self.wait()
         v = self.scratch(which)
         shape = (v.height, v.width, 4) if v.depth == 1 else (v.depth, v.height, v.width, 4)
-        out = np.empty(shape, dtype=np.float32)
+        # Binary16 texels (FB_FLAG_F16_STORAGE) are widened to float32.
+        out = np.empty(shape, dtype=np.float16 if v.texel_bytes == 8 else np.float32)
         check(load().fb_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(v.device_ptr),
                                    out.nbytes))
-        return out
+        return out.astype(np.float32, copy=False)
 
     def assert_ready(self) -> Atmosphere:
         """Consume; the scratch LUTs are released (:2209)."""

@@ -166,6 +166,34 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float w) {
 }
 __device__ __forceinline__ float4 ldg4(const float4* p) { return __ldg(p); }
 
+// Binary16 storage of the scattering LUT and the 3D scratch LUTs (the
+// reference's R16G16B16A16_SFLOAT images, src/precompute.rs:1215): DevParams
+// .flags bit kFlagF16Storage (= FB_FLAG_F16_STORAGE).  Those LUTs keep their
+// float4* type in every signature; with the bit set they hold 8-byte
+// binary16 texels at the same texel indices.
+constexpr int kFlagF16Storage = 0x2;
+typedef _Float16 Half4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 widen4(Half4 v) {
+    return make_float4(float(v.x), float(v.y), float(v.z), float(v.w));
+}
+// Round to nearest even, as an image store into a 16F image.
+typedef float Float4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ Half4 narrow4(float4 v) {
+    const Float4v f = {v.x, v.y, v.z, v.w};
+    return __builtin_convertvector(f, Half4);
+}
+// Texel i of a scattering-type LUT, binary16 when `h`.
+__device__ __forceinline__ float4 ld_tex(const float4* __restrict__ t, size_t i, bool h) {
+    if (h) return widen4(reinterpret_cast<const Half4*>(t)[i]);
+    return ldg4(t + i);
+}
+__device__ __forceinline__ void st_tex(float4* __restrict__ t, size_t i, float4 v, bool h) {
+    if (h)
+        reinterpret_cast<Half4*>(t)[i] = narrow4(v);
+    else
+        t[i] = v;
+}
+
 template <typename G>
 __device__ __forceinline__ float4 sample2d(const float4* __restrict__ tex, int w, int h, G u,
                                            G v) {
@@ -443,8 +471,9 @@ struct Taps3 {
 };
 
 // scattering.h:139-155: GetScattering on one texture, nu lerped manually
-// between two trilinear samples at slices floor(u_nu*(NU-1)) and +1.
-template <typename G>
+// between two trilinear samples at slices floor(u_nu*(NU-1)) and +1.  H: the
+// texture is binary16 (FB_FLAG_F16_STORAGE).
+template <typename G, bool H>
 __device__ __forceinline__ float4 get_scattering_tex(const DevParams& P,
                                                      const float4* __restrict__ tex, G r, G mu,
                                                      G mu_s, G nu, bool ground) {
@@ -460,19 +489,20 @@ __device__ __forceinline__ float4 get_scattering_tex(const DevParams& P,
     int W = P.s_nu * P.s_mus;
     Taps ty = linear_taps<G>(u_mu, P.s_mu), tz = linear_taps<G>(u_r, P.s_r);
     Taps tx0 = linear_taps<G>(x0, W), tx1 = linear_taps<G>(x1, W);
-    const float4* p00 = tex + (size_t(tz.i0) * P.s_mu + ty.i0) * W;
-    const float4* p01 = tex + (size_t(tz.i0) * P.s_mu + ty.i1) * W;
-    const float4* p10 = tex + (size_t(tz.i1) * P.s_mu + ty.i0) * W;
-    const float4* p11 = tex + (size_t(tz.i1) * P.s_mu + ty.i1) * W;
-    float4 a0 = lerp4(lerp4(lerp4(ldg4(p00 + tx0.i0), ldg4(p00 + tx0.i1), tx0.w),
-                            lerp4(ldg4(p01 + tx0.i0), ldg4(p01 + tx0.i1), tx0.w), ty.w),
-                      lerp4(lerp4(ldg4(p10 + tx0.i0), ldg4(p10 + tx0.i1), tx0.w),
-                            lerp4(ldg4(p11 + tx0.i0), ldg4(p11 + tx0.i1), tx0.w), ty.w),
+    const size_t p00 = (size_t(tz.i0) * P.s_mu + ty.i0) * W;
+    const size_t p01 = (size_t(tz.i0) * P.s_mu + ty.i1) * W;
+    const size_t p10 = (size_t(tz.i1) * P.s_mu + ty.i0) * W;
+    const size_t p11 = (size_t(tz.i1) * P.s_mu + ty.i1) * W;
+    auto at = [&](size_t row, int x) { return ld_tex(tex, row + x, H); };
+    float4 a0 = lerp4(lerp4(lerp4(at(p00, tx0.i0), at(p00, tx0.i1), tx0.w),
+                            lerp4(at(p01, tx0.i0), at(p01, tx0.i1), tx0.w), ty.w),
+                      lerp4(lerp4(at(p10, tx0.i0), at(p10, tx0.i1), tx0.w),
+                            lerp4(at(p11, tx0.i0), at(p11, tx0.i1), tx0.w), ty.w),
                       tz.w);
-    float4 a1 = lerp4(lerp4(lerp4(ldg4(p00 + tx1.i0), ldg4(p00 + tx1.i1), tx1.w),
-                            lerp4(ldg4(p01 + tx1.i0), ldg4(p01 + tx1.i1), tx1.w), ty.w),
-                      lerp4(lerp4(ldg4(p10 + tx1.i0), ldg4(p10 + tx1.i1), tx1.w),
-                            lerp4(ldg4(p11 + tx1.i0), ldg4(p11 + tx1.i1), tx1.w), ty.w),
+    float4 a1 = lerp4(lerp4(lerp4(at(p00, tx1.i0), at(p00, tx1.i1), tx1.w),
+                            lerp4(at(p01, tx1.i0), at(p01, tx1.i1), tx1.w), ty.w),
+                      lerp4(lerp4(at(p10, tx1.i0), at(p10, tx1.i1), tx1.w),
+                            lerp4(at(p11, tx1.i0), at(p11, tx1.i1), tx1.w), ty.w),
                       tz.w);
     return make_float4(a0.x * (1.0f - lerp) + a1.x * lerp, a0.y * (1.0f - lerp) + a1.y * lerp,
                        a0.z * (1.0f - lerp) + a1.z * lerp, a0.w * (1.0f - lerp) + a1.w * lerp);

@@ -41,6 +41,7 @@ struct BatchLaunch {
 struct BatchPlan {
     fb_builder* builder = nullptr;
     bool precise = false;
+    bool f16 = false;  // FB_FLAG_F16_STORAGE
     std::mutex mu;
     size_t n_total = 0;
     std::vector<fb_pending_atmosphere*> members;
@@ -66,8 +67,8 @@ struct BatchPlan {
 
     fb_status enqueue(hipStream_t s) const {
         for (const auto& l : launches)
-            FB_HIP_TRY(fbk::launch_batch(l.pass, l.order, precise, d_params, d_luts, l.d_tiles,
-                                         l.ntiles, l.threads, l.lds, l.grid, l.tpt, s));
+            FB_HIP_TRY(fbk::launch_batch(l.pass, l.order, precise, f16, d_params, d_luts,
+                                         l.d_tiles, l.ntiles, l.threads, l.lds, l.grid, l.tpt, s));
         return FB_OK;
     }
 };
@@ -168,7 +169,8 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
         l.ntiles = unsigned(tiles.size());
         if (plan.share_pct < 100) {
             const unsigned resident =
-                fbk::batch_resident_blocks(pass, order, plan.precise, l.threads, l.lds, l.tpt);
+                fbk::batch_resident_blocks(pass, order, plan.precise, plan.f16, l.threads, l.lds,
+                                           l.tpt);
             if (resident) l.grid = std::max(1u, resident * plan.share_pct / 100);
         }
         l.host_tiles = g.remap ? xcd_order(tiles) : std::move(tiles);
@@ -224,6 +226,7 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
     plan->builder = b;
     plan->share_pct = share_pct;
     plan->precise = (ps[0].flags & FB_FLAG_PRECISE) != 0;
+    plan->f16 = (ps[0].flags & FB_FLAG_F16_STORAGE) != 0;
     const size_t n = ps.size();
     std::vector<fbk::DevParams> dps(n);
     std::vector<fbk::LutPtrs> luts(n);
@@ -280,8 +283,9 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
                     last[i] = li;
         for (size_t li = 0; li < plan->launches.size(); ++li) {
             const BatchLaunch& l = plan->launches[li];
-            e = fbk::launch_batch(l.pass, l.order, plan->precise, plan->d_params, plan->d_luts,
-                                  l.d_tiles, l.ntiles, l.threads, l.lds, l.grid, l.tpt, s);
+            e = fbk::launch_batch(l.pass, l.order, plan->precise, plan->f16, plan->d_params,
+                                  plan->d_luts, l.d_tiles, l.ntiles, l.threads, l.lds, l.grid,
+                                  l.tpt, s);
             for (size_t i = 0; i < n && e == hipSuccess; ++i) {
                 if (last[i] != li) continue;
                 e = hipEventRecord(plan->members[i]->done, s);
@@ -382,13 +386,14 @@ fb_status build_batch(fb_builder* b, hipStream_t s, const fb_parameters* params,
         out[i] = nullptr;
     }
     FB_HIP_TRY(hipSetDevice(b->device));
-    // Groups: one per precision mode, at most 256 atmospheres each (the tile
-    // word's 8-bit atmosphere slot).
+    // Groups: one per precision and storage mode, at most 256 atmospheres
+    // each (the tile word's 8-bit atmosphere slot).
     std::vector<size_t> built;
-    for (uint32_t flags : {0u, FB_FLAG_PRECISE}) {
+    const uint32_t kModes = FB_FLAG_PRECISE | FB_FLAG_F16_STORAGE;
+    for (uint32_t flags : {0u, FB_FLAG_PRECISE, FB_FLAG_F16_STORAGE, kModes}) {
         std::vector<size_t> idx;
         for (size_t i = 0; i < n; ++i)
-            if ((params[i].flags & FB_FLAG_PRECISE) == flags) idx.push_back(i);
+            if ((params[i].flags & kModes) == flags) idx.push_back(i);
         for (size_t g0 = 0; g0 < idx.size(); g0 += 256) {
             size_t g1 = std::min(idx.size(), g0 + 256);
             std::vector<fb_parameters> ps;

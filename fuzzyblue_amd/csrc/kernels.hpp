@@ -108,13 +108,14 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
 // blocks walk the tile list (0 = one block per tile).
 // tpt: multiple scattering's texels per thread (1, 2 or 4; every row of the
 // launch must fit threads x tpt texels).
-hipError_t launch_batch(int pass, int order, bool precise, const DevParams* d_params,
+// f16: the set's scattering-type LUTs are binary16 (FB_FLAG_F16_STORAGE).
+hipError_t launch_batch(int pass, int order, bool precise, bool f16, const DevParams* d_params,
                         const LutPtrs* d_luts, const uint32_t* d_tiles, unsigned ntiles,
                         unsigned threads, size_t lds, unsigned grid_blocks, unsigned tpt,
                         hipStream_t s);
 // Blocks of a batched launch of `pass` resident on the device at once.
-unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threads, size_t lds,
-                               unsigned tpt);
+unsigned batch_resident_blocks(int pass, int order, bool precise, bool f16, unsigned threads,
+                               size_t lds, unsigned tpt);
 
 // OD: optical-depth LUT written beside T (precision mode), or null.
 hipError_t launch_transmittance(const DevParams& P, float4* T, float4* OD, hipStream_t s);

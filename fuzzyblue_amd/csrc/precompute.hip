@@ -7,6 +7,7 @@
 // wrappers at the bottom take a DevParams by value and raw device pointers;
 // the host runtime (runtime.hip) owns allocation and ordering.
 #include <math.h>
+#include <type_traits>
 
 #include "atmo_device.hpp"
 #include "kernels.hpp"
@@ -341,7 +342,7 @@ __device__ __forceinline__ float3 lerp3(float3 a, float3 b, float w) {
     return make_float3(fmaf(w, b.x - a.x, a.x), fmaf(w, b.y - a.y, a.y), fmaf(w, b.z - a.z, a.z));
 }
 
-template <typename G, bool COMP>
+template <typename G, bool COMP, bool H>
 __device__ __forceinline__ void single_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                            const float4* __restrict__ T,
                                                            const float4* __restrict__ SR,
@@ -428,9 +429,9 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
                                  ms[1].value() * dxf * P.solar[1] * P.mie_scattering[1],
                                  ms[2].value() * dxf * P.solar[2] * P.mie_scattering[2]);
         const size_t idx = (size_t(z) * MU + y) * W + x;
-        dR[idx] = make_float4(ray.x, ray.y, ray.z, 0.0f);
-        dM[idx] = make_float4(mie.x, mie.y, mie.z, 0.0f);
-        S[idx] = make_float4(ray.x, ray.y, ray.z, mie.x);
+        st_tex(dR, idx, make_float4(ray.x, ray.y, ray.z, 0.0f), H);
+        st_tex(dM, idx, make_float4(mie.x, mie.y, mie.z, 0.0f), H);
+        st_tex(S, idx, make_float4(ray.x, ray.y, ray.z, mie.x), H);
     }
 }
 
@@ -455,18 +456,21 @@ __device__ __forceinline__ Taps slice_taps(const DevParams& P, int slice, G u_mu
     return linear_taps<G>(x, P.s_nu * P.s_mus);
 }
 
-__device__ __forceinline__ float4 trilinear_taps(const DevParams& P,
-                                                 const float4* __restrict__ tex, const Taps& tx,
-                                                 const Taps& ty, const Taps& tz) {
+// H: the texture is binary16 (FB_FLAG_F16_STORAGE).
+template <bool H>
+__device__ __forceinline__ float4 trilinear_taps_t(const DevParams& P,
+                                                   const float4* __restrict__ tex, const Taps& tx,
+                                                   const Taps& ty, const Taps& tz) {
     const int W = P.s_nu * P.s_mus;
-    const float4* p00 = tex + (size_t(tz.i0) * P.s_mu + ty.i0) * W;
-    const float4* p01 = tex + (size_t(tz.i0) * P.s_mu + ty.i1) * W;
-    const float4* p10 = tex + (size_t(tz.i1) * P.s_mu + ty.i0) * W;
-    const float4* p11 = tex + (size_t(tz.i1) * P.s_mu + ty.i1) * W;
-    return lerp4(lerp4(lerp4(ldg4(p00 + tx.i0), ldg4(p00 + tx.i1), tx.w),
-                       lerp4(ldg4(p01 + tx.i0), ldg4(p01 + tx.i1), tx.w), ty.w),
-                 lerp4(lerp4(ldg4(p10 + tx.i0), ldg4(p10 + tx.i1), tx.w),
-                       lerp4(ldg4(p11 + tx.i0), ldg4(p11 + tx.i1), tx.w), ty.w),
+    const size_t p00 = (size_t(tz.i0) * P.s_mu + ty.i0) * W;
+    const size_t p01 = (size_t(tz.i0) * P.s_mu + ty.i1) * W;
+    const size_t p10 = (size_t(tz.i1) * P.s_mu + ty.i0) * W;
+    const size_t p11 = (size_t(tz.i1) * P.s_mu + ty.i1) * W;
+    auto at = [&](size_t row, int x) { return ld_tex(tex, row + x, H); };
+    return lerp4(lerp4(lerp4(at(p00, tx.i0), at(p00, tx.i1), tx.w),
+                       lerp4(at(p01, tx.i0), at(p01, tx.i1), tx.w), ty.w),
+                 lerp4(lerp4(at(p10, tx.i0), at(p10, tx.i1), tx.w),
+                       lerp4(at(p11, tx.i0), at(p11, tx.i1), tx.w), ty.w),
                  tz.w);
 }
 
@@ -695,7 +699,7 @@ __device__ __forceinline__ void density_geometry_block(const DevParams& P, const
 // Per-sample body of the density quadrature for one pair of directions
 // (phi_m, phi_{31-m}): both share cos(phi), hence nu2 and the two phase
 // functions; their sin(phi) terms have opposite signs.
-template <typename G, bool SINGLE>
+template <typename G, bool SINGLE, bool H>
 __device__ __forceinline__ void scattering_density_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ DG, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J) {
@@ -754,7 +758,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
             int rem = e % (kDenTheta * nslots);
             int l = rem / nslots, slot = rem % nslots;
             const float4* tex = SINGLE ? (tex_sel == 0 ? dR : dM) : dMS;
-            s_raw[e] = trilinear_taps(P, tex, s_tx[slot], s_ty[l], tz);
+            s_raw[e] = trilinear_taps_t<H>(P, tex, s_tx[slot], s_ty[l], tz);
         }
     }
     __syncthreads();
@@ -852,7 +856,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         acc.y = fmaf(cm1 * dom, aM.y, fmaf(cr1 * dom, aR.y, acc.y));
         acc.z = fmaf(cm2 * dom, aM.z, fmaf(cr2 * dom, aR.z, acc.z));
     }
-    J[(size_t(z) * MU + y) * W + x] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    st_tex(J, (size_t(z) * MU + y) * W + x, make_float4(acc.x, acc.y, acc.z, 0.0f), H);
 }
 
 
@@ -870,7 +874,7 @@ __host__ __device__ inline size_t indirect_lds_bytes_per_wave(int nu, bool singl
            sizeof(ProfEntry) * size_t(ntex) * kIndTheta * (nu + 1);
 }
 
-template <typename G, bool SINGLE>
+template <typename G, bool SINGLE, bool H>
 __device__ __forceinline__ void indirect_irradiance_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ dR, const float4* __restrict__ dM,
     const float4* __restrict__ dMS, float4* __restrict__ dE, float4* __restrict__ E) {
     extern __shared__ float4 s_dyn[];
@@ -899,7 +903,7 @@ __device__ __forceinline__ void indirect_irradiance_block(const DevParams& P, co
         G ct = ind_cos<G>(jj);
         Taps ty = linear_taps<G>(scattering_u_mu<G>(P, r, ct, false), P.s_mu);
         const float4* tex = SINGLE ? (tex_sel == 0 ? dR : dM) : dMS;
-        s_raw[e] = trilinear_taps(P, tex, slice_taps<G>(P, slot - 1, u_mu_s), ty, tz);
+        s_raw[e] = trilinear_taps_t<H>(P, tex, slice_taps<G>(P, slot - 1, u_mu_s), ty, tz);
     }
     __syncthreads();
     for (int e = lane; e < NTEX * kIndTheta * nent; e += 64) {
@@ -972,7 +976,15 @@ struct Rgb {
     float x, y, z;
 };
 
-template <typename G, bool COMP, int TPT>
+__device__ __forceinline__ Rgb to_rgb(const Rgb& v) { return v; }
+__device__ __forceinline__ Rgb to_rgb(const Half4& v) {
+    return Rgb{float(v.x), float(v.y), float(v.z)};
+}
+
+// H: the density LUT J, the delta and the scattering LUT are binary16
+// (FB_FLAG_F16_STORAGE); the density rows then stay binary16 in the slots
+// until put() widens them.
+template <typename G, bool COMP, int TPT, bool H>
 __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                              const float4* __restrict__ MR,
                                                              const float4* __restrict__ J,
@@ -1037,23 +1049,30 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
     // The four density-row slots of this thread's texels x = tid + q*nt,
     // carried from sample to sample (ms_row_slots): each sample loads only
     // the rows it does not share with the previous one.
-    Rgb slot[TPT][4];
+    typedef typename std::conditional<H, Half4, Rgb>::type Slot;
+    Slot slot[TPT][4];
 #pragma unroll
     for (int q = 0; q < TPT; ++q)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) slot[q][k] = Rgb{0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < 4; ++k) slot[q][k] = Slot{};
     // Row loads as buffer loads: the row start (block-uniform) goes in the
     // scalar offset and the texel's byte offset in a loop-invariant vector
     // offset, so no address arithmetic writes a register between the loads
     // and each load lands directly in its slot.  Loads past the LUT's end
     // (texels x >= W of the last row) return 0 and are never used.
-    const __amdgpu_buffer_rsrc_t jb = buffer_rsrc(J, int(size_t(P.s_r) * MU * W * 16));
+    constexpr int TB = H ? 8 : 16;  // bytes per texel of J
+    const __amdgpu_buffer_rsrc_t jb = buffer_rsrc(J, int(size_t(P.s_r) * MU * W * TB));
     int xoff[TPT];
 #pragma unroll
-    for (int q = 0; q < TPT; ++q) xoff[q] = (tid + q * nt) * 16;
+    for (int q = 0; q < TPT; ++q) xoff[q] = (tid + q * nt) * TB;
     auto ld = [&](int o, int q) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b96(jb, xoff[q], o * 16, 0);
-        return Rgb{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
+        if constexpr (H) {
+            return __builtin_bit_cast(Half4,
+                                      __builtin_amdgcn_raw_buffer_load_b64(jb, xoff[q], o * TB, 0));
+        } else {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b96(jb, xoff[q], o * TB, 0);
+            return Rgb{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
+        }
     };
     // Issue sample i's loads; they stay in flight until put() reads them.
     auto advance = [&](int i) {
@@ -1082,8 +1101,9 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
     }
     // The (mu, r)-bilinear density value of sample i at texel x (the slot
     // weights are the bilinear weights), into the staged row dst.
-    auto put = [&](int i, const Rgb* t, Rgb* dst, int x) {
+    auto put = [&](int i, const Slot* s, Rgb* dst, int x) {
         const float4 w = d_wt(i);
+        const Rgb t[4] = {to_rgb(s[0]), to_rgb(s[1]), to_rgb(s[2]), to_rgb(s[3])};
         const Rgb v{fmaf(w.w, t[3].x, fmaf(w.z, t[2].x, fmaf(w.y, t[1].x, w.x * t[0].x))),
                     fmaf(w.w, t[3].y, fmaf(w.z, t[2].y, fmaf(w.y, t[1].y, w.x * t[0].y))),
                     fmaf(w.w, t[3].z, fmaf(w.z, t[2].z, fmaf(w.y, t[1].z, w.x * t[0].z)))};
@@ -1152,11 +1172,13 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         if (x < W) {
             const size_t idx = (size_t(z) * MU + y) * W + x;
             float m0 = acc[q][0].value(), m1 = acc[q][1].value(), m2 = acc[q][2].value();
-            dMS[idx] = make_float4(m0, m1, m2, 0.0f);
+            st_tex(dMS, idx, make_float4(m0, m1, m2, 0.0f), H);
             float pr = phase_rayleigh(tnu[q]);
-            float4 prev = S[idx];
-            S[idx] = make_float4(m0 / pr + prev.x, m1 / pr + prev.y, m2 / pr + prev.z,
-                                 0.0f + prev.w);
+            float4 prev = ld_tex(S, idx, H);
+            st_tex(S, idx,
+                   make_float4(m0 / pr + prev.x, m1 / pr + prev.y, m2 / pr + prev.z,
+                               0.0f + prev.w),
+                   H);
         }
     }
 }
@@ -1241,7 +1263,7 @@ __device__ __forceinline__ unsigned logical_block(bool remap) {
     return remap ? xcd_remap(L, gridDim.x * gridDim.y * gridDim.z) : L;
 }
 
-template <typename G, bool COMP>
+template <typename G, bool COMP, bool H>
 __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const float4* __restrict__ T,
                                                            const float4* __restrict__ SR,
                                                            float4* __restrict__ dR,
@@ -1249,9 +1271,9 @@ __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const fl
                                                            float4* __restrict__ S,
                                                            unsigned remap) {
     const unsigned b = logical_block(remap);
-    single_scattering_block<G, COMP>(P, b % P.s_mu, b / P.s_mu, 0, T, SR, dR, dM, S);
+    single_scattering_block<G, COMP, H>(P, b % P.s_mu, b / P.s_mu, 0, T, SR, dR, dM, S);
 }
-template <typename G, bool COMP>
+template <typename G, bool COMP, bool H>
 __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
                                                             const LutPtrs* __restrict__ L,
                                                             const uint32_t* __restrict__ tiles, unsigned ntiles) {
@@ -1264,19 +1286,19 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs l = lut_ptrs(L, atm);
-        single_scattering_block<G, COMP>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.SR, l.dR,
+        single_scattering_block<G, COMP, H>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.SR, l.dR,
                                          l.dM, l.S);
     }
 }
 
-template <typename G, bool SINGLE>
+template <typename G, bool SINGLE, bool H>
 __global__ __launch_bounds__(256, 8) void k_scattering_density(
     DevParams P, const float4* __restrict__ DG, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J, unsigned remap) {
     const unsigned b = logical_block(remap), gx = gridDim.x;
-    scattering_density_block<G, SINGLE>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus), DG,
-                                        dR, dM, dMS, dE, J);
+    scattering_density_block<G, SINGLE, H>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus),
+                                           DG, dR, dM, dMS, dE, J);
 }
 
 // Per-set geometry: blocks [0, R + MUS) fill the density geometry table,
@@ -1314,7 +1336,7 @@ __global__ __launch_bounds__(64) void kb_geometry(const DevParams* __restrict__ 
         geometry_block<G>(Ps[atm], local, l.T, l.OD, l.DG, l.SR, l.MR);
     }
 }
-template <typename G, bool SINGLE>
+template <typename G, bool SINGLE, bool H>
 __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams* __restrict__ Ps,
                                                                 const LutPtrs* __restrict__ L,
                                                                 const uint32_t* __restrict__ tiles, unsigned ntiles) {
@@ -1328,19 +1350,19 @@ __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams*
         const DevParams& P = Ps[atm];
         const LutPtrs l = lut_ptrs(L, atm);
         const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
-        scattering_density_block<G, SINGLE>(P, local % gx, (local / gx) % P.s_mus,
-                                            local / (gx * P.s_mus), l.DG, l.dR, l.dM, l.dMS, l.dE,
-                                            l.J);
+        scattering_density_block<G, SINGLE, H>(P, local % gx, (local / gx) % P.s_mus,
+                                               local / (gx * P.s_mus), l.DG, l.dR, l.dM, l.dMS,
+                                               l.dE, l.J);
     }
 }
 
-template <typename G, bool SINGLE>
+template <typename G, bool SINGLE, bool H>
 __global__ __launch_bounds__(64 * kIndWaves) void k_indirect_irradiance(
     DevParams P, const float4* __restrict__ dR, const float4* __restrict__ dM,
     const float4* __restrict__ dMS, float4* __restrict__ dE, float4* __restrict__ E) {
-    indirect_irradiance_block<G, SINGLE>(P, blockIdx.x, 0, 0, dR, dM, dMS, dE, E);
+    indirect_irradiance_block<G, SINGLE, H>(P, blockIdx.x, 0, 0, dR, dM, dMS, dE, E);
 }
-template <typename G, bool SINGLE>
+template <typename G, bool SINGLE, bool H>
 __global__ __launch_bounds__(64 * kIndWaves) void kb_indirect_irradiance(
     const DevParams* __restrict__ Ps, const LutPtrs* __restrict__ L,
     const uint32_t* __restrict__ tiles, unsigned ntiles) {
@@ -1352,11 +1374,12 @@ __global__ __launch_bounds__(64 * kIndWaves) void kb_indirect_irradiance(
         int atm, local;
         decode_tile(tiles[t], atm, local);
         const LutPtrs l = lut_ptrs(L, atm);
-        indirect_irradiance_block<G, SINGLE>(Ps[atm], local, 0, 0, l.dR, l.dM, l.dMS, l.dE, l.E);
+        indirect_irradiance_block<G, SINGLE, H>(Ps[atm], local, 0, 0, l.dR, l.dM, l.dMS, l.dE,
+                                                l.E);
     }
 }
 
-template <typename G, bool COMP, int TPT>
+template <typename G, bool COMP, int TPT, bool H>
 __global__ __launch_bounds__(kMsMaxThreads) void k_multiple_scattering(DevParams P,
                                                              const float4* __restrict__ MR,
                                                              const float4* __restrict__ J,
@@ -1364,9 +1387,9 @@ __global__ __launch_bounds__(kMsMaxThreads) void k_multiple_scattering(DevParams
                                                              float4* __restrict__ S,
                                                              unsigned remap) {
     const unsigned b = logical_block(remap);
-    multiple_scattering_block<G, COMP, TPT>(P, b % P.s_mu, b / P.s_mu, 0, MR, J, dMS, S);
+    multiple_scattering_block<G, COMP, TPT, H>(P, b % P.s_mu, b / P.s_mu, 0, MR, J, dMS, S);
 }
-template <typename G, bool COMP, int TPT>
+template <typename G, bool COMP, int TPT, bool H>
 __global__ __launch_bounds__(kMsMaxThreads) void kb_multiple_scattering(const DevParams* __restrict__ Ps,
                                                               const LutPtrs* __restrict__ L,
                                                               const uint32_t* __restrict__ tiles, unsigned ntiles) {
@@ -1379,8 +1402,8 @@ __global__ __launch_bounds__(kMsMaxThreads) void kb_multiple_scattering(const De
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs l = lut_ptrs(L, atm);
-        multiple_scattering_block<G, COMP, TPT>(P, local % P.s_mu, local / P.s_mu, 0, l.MR, l.J,
-                                                l.dMS, l.S);
+        multiple_scattering_block<G, COMP, TPT, H>(P, local % P.s_mu, local / P.s_mu, 0, l.MR,
+                                                   l.J, l.dMS, l.S);
     }
 }
 
@@ -1438,23 +1461,32 @@ hipError_t launch_geometry(const DevParams& P, const float4* T, const float4* OD
     return hipGetLastError();
 }
 
-hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* SR,
-                                    float4* dR, float4* dM, float4* S, const Tuning& tu,
-                                    hipStream_t s) {
+template <bool H>
+static hipError_t launch_single_scattering_h(const DevParams& P, const float4* T,
+                                             const float4* SR, float4* dR, float4* dM, float4* S,
+                                             const Tuning& tu, hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassSingle, 2, tu);
     const unsigned remap = (tu.remap >> kPassSingle) & 1u;
     dim3 grid(P.s_mu, P.s_r);
     if (P.flags & 1)
-        k_single_scattering<double, false><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
+        k_single_scattering<double, false, H><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
     else
-        k_single_scattering<float, false><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
+        k_single_scattering<float, false, H><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
     return hipGetLastError();
 }
+hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* SR,
+                                    float4* dR, float4* dM, float4* S, const Tuning& tu,
+                                    hipStream_t s) {
+    return (P.flags & kFlagF16Storage)
+               ? launch_single_scattering_h<true>(P, T, SR, dR, dM, S, tu, s)
+               : launch_single_scattering_h<false>(P, T, SR, dR, dM, S, tu, s);
+}
 
-hipError_t launch_scattering_density(const DevParams& P, int order, const float4* DG,
-                                     const float4* dR, const float4* dM, const float4* dMS,
-                                     const float4* dE, float4* J, const Tuning& tu,
-                                     hipStream_t s) {
+template <bool H>
+static hipError_t launch_scattering_density_h(const DevParams& P, int order, const float4* DG,
+                                              const float4* dR, const float4* dM,
+                                              const float4* dMS, const float4* dE, float4* J,
+                                              const Tuning& tu, hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassDensity, order, tu);
     const unsigned remap = (tu.remap >> kPassDensity) & 1u;
     dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, threads), P.s_mus, P.s_r);
@@ -1464,19 +1496,28 @@ hipError_t launch_scattering_density(const DevParams& P, int order, const float4
     size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
     const bool precise = P.flags & 1;
     if (precise && single)
-        k_scattering_density<double, true><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<double, true, H><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     else if (precise)
-        k_scattering_density<double, false><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<double, false, H><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     else if (single)
-        k_scattering_density<float, true><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<float, true, H><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     else
-        k_scattering_density<float, false><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        k_scattering_density<float, false, H><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     return hipGetLastError();
 }
+hipError_t launch_scattering_density(const DevParams& P, int order, const float4* DG,
+                                     const float4* dR, const float4* dM, const float4* dMS,
+                                     const float4* dE, float4* J, const Tuning& tu,
+                                     hipStream_t s) {
+    return (P.flags & kFlagF16Storage)
+               ? launch_scattering_density_h<true>(P, order, DG, dR, dM, dMS, dE, J, tu, s)
+               : launch_scattering_density_h<false>(P, order, DG, dR, dM, dMS, dE, J, tu, s);
+}
 
-hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float4* dR,
-                                      const float4* dM, const float4* dMS, float4* dE, float4* E,
-                                      const Tuning& tu, hipStream_t s) {
+template <bool H>
+static hipError_t launch_indirect_irradiance_h(const DevParams& P, int order, const float4* dR,
+                                               const float4* dM, const float4* dMS, float4* dE,
+                                               float4* E, const Tuning& tu, hipStream_t s) {
     size_t n = size_t(P.i_mus) * P.i_r;
     const bool single = order == 1;
     size_t per_wave = indirect_lds_bytes_per_wave(P.s_nu, single);
@@ -1485,18 +1526,27 @@ hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float
     dim3 grid(cdiv(n, waves)), block(64 * waves);
     const bool precise = P.flags & 1;
     if (precise && single)
-        k_indirect_irradiance<double, true><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
+        k_indirect_irradiance<double, true, H><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
     else if (precise)
-        k_indirect_irradiance<double, false><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
+        k_indirect_irradiance<double, false, H><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
     else if (single)
-        k_indirect_irradiance<float, true><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
+        k_indirect_irradiance<float, true, H><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
     else
-        k_indirect_irradiance<float, false><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
+        k_indirect_irradiance<float, false, H><<<grid, block, lds, s>>>(P, dR, dM, dMS, dE, E);
     return hipGetLastError();
 }
+hipError_t launch_indirect_irradiance(const DevParams& P, int order, const float4* dR,
+                                      const float4* dM, const float4* dMS, float4* dE, float4* E,
+                                      const Tuning& tu, hipStream_t s) {
+    return (P.flags & kFlagF16Storage)
+               ? launch_indirect_irradiance_h<true>(P, order, dR, dM, dMS, dE, E, tu, s)
+               : launch_indirect_irradiance_h<false>(P, order, dR, dM, dMS, dE, E, tu, s);
+}
 
-hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, const float4* J,
-                                      float4* dMS, float4* S, const Tuning& tu, hipStream_t s) {
+template <bool H>
+static hipError_t launch_multiple_scattering_h(const DevParams& P, const float4* MR,
+                                               const float4* J, float4* dMS, float4* S,
+                                               const Tuning& tu, hipStream_t s) {
     const int W = P.s_nu * P.s_mus;
     const int threads = int(tuned_threads(P, kPassMultiple, 2, tu));
     const unsigned remap = (tu.remap >> kPassMultiple) & 1u;
@@ -1508,13 +1558,19 @@ hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, cons
     const int tpt = W <= threads ? 1 : (W <= 2 * threads ? 2 : 4);
     const bool precise = P.flags & 1;
 #define FB_MS(G, TPT) \
-    k_multiple_scattering<G, false, TPT><<<grid, threads, lds, s>>>(P, MR, J, dMS, S, remap)
+    k_multiple_scattering<G, false, TPT, H><<<grid, threads, lds, s>>>(P, MR, J, dMS, S, remap)
     if (precise)
         tpt == 1 ? FB_MS(double, 1) : tpt == 2 ? FB_MS(double, 2) : FB_MS(double, 4);
     else
         tpt == 1 ? FB_MS(float, 1) : tpt == 2 ? FB_MS(float, 2) : FB_MS(float, 4);
 #undef FB_MS
     return hipGetLastError();
+}
+hipError_t launch_multiple_scattering(const DevParams& P, const float4* MR, const float4* J,
+                                      float4* dMS, float4* S, const Tuning& tu, hipStream_t s) {
+    return (P.flags & kFlagF16Storage)
+               ? launch_multiple_scattering_h<true>(P, MR, J, dMS, S, tu, s)
+               : launch_multiple_scattering_h<false>(P, MR, J, dMS, S, tu, s);
 }
 
 // ------------------------------------------------------------ tuning
@@ -1602,10 +1658,11 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
     return {0, 0, 0};
 }
 
-hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
-                        const LutPtrs* L, const uint32_t* tiles, unsigned ntiles,
-                        unsigned threads, size_t lds, unsigned grid_blocks, unsigned tpt,
-                        hipStream_t s) {
+template <bool H>
+static hipError_t launch_batch_h(int pass, int order, bool precise, const DevParams* Ps,
+                                 const LutPtrs* L, const uint32_t* tiles, unsigned ntiles,
+                                 unsigned threads, size_t lds, unsigned grid_blocks, unsigned tpt,
+                                 hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
     const dim3 grid(grid_blocks && grid_blocks < ntiles ? grid_blocks : ntiles), block(threads);
     const bool single = (order - 1) == 1;
@@ -1624,9 +1681,9 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
             break;
         case kPassSingle:
             if (precise)
-                kb_single_scattering<double, false><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+                kb_single_scattering<double, false, H><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_single_scattering<float, false><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+                kb_single_scattering<float, false, H><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassClear:
             kb_clear_irradiance<<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
@@ -1639,26 +1696,26 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
             break;
         case kPassDensity:
             if (precise && single)
-                kb_scattering_density<double, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_scattering_density<double, true, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (precise)
-                kb_scattering_density<double, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_scattering_density<double, false, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (single)
-                kb_scattering_density<float, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_scattering_density<float, true, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_scattering_density<float, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_scattering_density<float, false, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassIndirect:
             if (precise && single)
-                kb_indirect_irradiance<double, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_indirect_irradiance<double, true, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (precise)
-                kb_indirect_irradiance<double, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_indirect_irradiance<double, false, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (single)
-                kb_indirect_irradiance<float, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_indirect_irradiance<float, true, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_indirect_irradiance<float, false><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                kb_indirect_irradiance<float, false, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassMultiple:
-#define FB_KBMS(G, T) kb_multiple_scattering<G, false, T><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles)
+#define FB_KBMS(G, T) kb_multiple_scattering<G, false, T, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles)
             if (precise)
                 tpt == 1 ? FB_KBMS(double, 1) : tpt == 2 ? FB_KBMS(double, 2) : FB_KBMS(double, 4);
             else
@@ -1670,11 +1727,21 @@ hipError_t launch_batch(int pass, int order, bool precise, const DevParams* Ps,
     }
     return hipGetLastError();
 }
+hipError_t launch_batch(int pass, int order, bool precise, bool f16, const DevParams* Ps,
+                        const LutPtrs* L, const uint32_t* tiles, unsigned ntiles,
+                        unsigned threads, size_t lds, unsigned grid_blocks, unsigned tpt,
+                        hipStream_t s) {
+    return f16 ? launch_batch_h<true>(pass, order, precise, Ps, L, tiles, ntiles, threads, lds,
+                                      grid_blocks, tpt, s)
+               : launch_batch_h<false>(pass, order, precise, Ps, L, tiles, ntiles, threads, lds,
+                                       grid_blocks, tpt, s);
+}
 
 // Blocks of one batched launch that fit on the device at once (per-CU
 // occupancy of the launch's kernel x CUs); 0 if it cannot be determined.
-unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threads, size_t lds,
-                               unsigned tpt) {
+template <bool H>
+static unsigned batch_resident_blocks_h(int pass, int order, bool precise, unsigned threads,
+                                        size_t lds, unsigned tpt) {
     const bool single = (order - 1) == 1;
     const void* fn = nullptr;
     switch (pass) {
@@ -1687,8 +1754,8 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
                          : (const void*)kb_direct_irradiance<float>;
             break;
         case kPassSingle:
-            fn = precise ? (const void*)kb_single_scattering<double, false>
-                         : (const void*)kb_single_scattering<float, false>;
+            fn = precise ? (const void*)kb_single_scattering<double, false, H>
+                         : (const void*)kb_single_scattering<float, false, H>;
             break;
         case kPassClear:
             fn = (const void*)kb_clear_irradiance;
@@ -1697,27 +1764,27 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
             fn = precise ? (const void*)kb_geometry<double> : (const void*)kb_geometry<float>;
             break;
         case kPassDensity:
-            fn = precise ? (single ? (const void*)kb_scattering_density<double, true>
-                                   : (const void*)kb_scattering_density<double, false>)
-                         : (single ? (const void*)kb_scattering_density<float, true>
-                                   : (const void*)kb_scattering_density<float, false>);
+            fn = precise ? (single ? (const void*)kb_scattering_density<double, true, H>
+                                   : (const void*)kb_scattering_density<double, false, H>)
+                         : (single ? (const void*)kb_scattering_density<float, true, H>
+                                   : (const void*)kb_scattering_density<float, false, H>);
             break;
         case kPassIndirect:
-            fn = precise ? (single ? (const void*)kb_indirect_irradiance<double, true>
-                                   : (const void*)kb_indirect_irradiance<double, false>)
-                         : (single ? (const void*)kb_indirect_irradiance<float, true>
-                                   : (const void*)kb_indirect_irradiance<float, false>);
+            fn = precise ? (single ? (const void*)kb_indirect_irradiance<double, true, H>
+                                   : (const void*)kb_indirect_irradiance<double, false, H>)
+                         : (single ? (const void*)kb_indirect_irradiance<float, true, H>
+                                   : (const void*)kb_indirect_irradiance<float, false, H>);
             break;
         case kPassMultiple:
             if (tpt == 1)
-                fn = precise ? (const void*)kb_multiple_scattering<double, false, 1>
-                             : (const void*)kb_multiple_scattering<float, false, 1>;
+                fn = precise ? (const void*)kb_multiple_scattering<double, false, 1, H>
+                             : (const void*)kb_multiple_scattering<float, false, 1, H>;
             else if (tpt == 2)
-                fn = precise ? (const void*)kb_multiple_scattering<double, false, 2>
-                             : (const void*)kb_multiple_scattering<float, false, 2>;
+                fn = precise ? (const void*)kb_multiple_scattering<double, false, 2, H>
+                             : (const void*)kb_multiple_scattering<float, false, 2, H>;
             else
-                fn = precise ? (const void*)kb_multiple_scattering<double, false, 4>
-                             : (const void*)kb_multiple_scattering<float, false, 4>;
+                fn = precise ? (const void*)kb_multiple_scattering<double, false, 4, H>
+                             : (const void*)kb_multiple_scattering<float, false, 4, H>;
             break;
     }
     int per_cu = 0, dev = 0, cus = 0;
@@ -1729,6 +1796,11 @@ unsigned batch_resident_blocks(int pass, int order, bool precise, unsigned threa
         return 0;
     }
     return unsigned(per_cu) * unsigned(cus);
+}
+unsigned batch_resident_blocks(int pass, int order, bool precise, bool f16, unsigned threads,
+                               size_t lds, unsigned tpt) {
+    return f16 ? batch_resident_blocks_h<true>(pass, order, precise, threads, lds, tpt)
+               : batch_resident_blocks_h<false>(pass, order, precise, threads, lds, tpt);
 }
 
 hipError_t launch_fill_zero(float4* p, size_t n, hipStream_t s) {

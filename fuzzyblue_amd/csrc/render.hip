@@ -23,17 +23,21 @@ __device__ __forceinline__ float3 extrapolated_single_mie(const DevParams& P, fl
                        s.z * s.w / s.x * k * (P.mie_scattering[2] / P.rayleigh_scattering[2]));
 }
 
-// render_sky.h:21-43 (GLSL mix of the two nu slices)
+// render_sky.h:21-43 (GLSL mix of the two nu slices).  H: the scattering LUT
+// is binary16 (FB_FLAG_F16_STORAGE); every function and kernel below that
+// reads it takes the same parameter.
+template <bool H>
 __device__ __forceinline__ float4 combined_scattering(const DevParams& P,
                                                       const float4* __restrict__ S, float r,
                                                       float mu, float mu_s, float nu, bool ground,
                                                       float3& mie) {
-    float4 c = get_scattering_tex<float>(P, S, r, mu, mu_s, nu, ground);
+    float4 c = get_scattering_tex<float, H>(P, S, r, mu, mu_s, nu, ground);
     mie = extrapolated_single_mie(P, c);
     return c;
 }
 
 // render_sky.h:45-109
+template <bool H>
 __device__ void sky_radiance(const DevParams& P, const float4* __restrict__ T,
                              const float4* __restrict__ S, float3 camera, float3 view,
                              float3 sun, float3& rad, float3& trans) {
@@ -56,12 +60,13 @@ __device__ void sky_radiance(const DevParams& P, const float4* __restrict__ T,
     bool ground = ray_intersects_ground<float>(P, r, mu);
     trans = ground ? make_float3(0.0f, 0.0f, 0.0f) : transmittance_to_top<float>(P, T, r, mu);
     float3 mie;
-    float4 sc = combined_scattering(P, S, r, mu, mu_s, nu, ground, mie);
+    float4 sc = combined_scattering<H>(P, S, r, mu, mu_s, nu, ground, mie);
     float pr = phase_rayleigh(nu), pm = phase_mie(P.g, nu);
     rad = make_float3(sc.x * pr + mie.x * pm, sc.y * pr + mie.y * pm, sc.z * pr + mie.z * pm);
 }
 
 // render_sky.h:111-191
+template <bool H>
 __device__ void sky_radiance_to_point(const DevParams& P, const float4* __restrict__ T,
                                       const float4* __restrict__ S, float3 camera, float3 view,
                                       float3 point, float3 sun, float3& rad, float3& trans) {
@@ -86,13 +91,13 @@ __device__ void sky_radiance_to_point(const DevParams& P, const float4* __restri
     bool ground = ray_intersects_ground<float>(P, r, mu);
     trans = get_transmittance<float>(P, T, r, mu, d, ground);
     float3 mie;
-    float4 sc = combined_scattering(P, S, r, mu, mu_s, nu, ground, mie);
+    float4 sc = combined_scattering<H>(P, S, r, mu, mu_s, nu, ground, mie);
     if (!isinf(d)) {
         float r_p = clamp_radius<float>(P, sqrtf(d * d + 2.0f * r * mu * d + r * r));
         float mu_p = (r * mu + d) / r_p;
         float mu_s_p = (r * mu_s + d * nu) / r_p;
         float3 mie_p;
-        float4 sc_p = combined_scattering(P, S, r_p, mu_p, mu_s_p, nu, ground, mie_p);
+        float4 sc_p = combined_scattering<H>(P, S, r_p, mu_p, mu_s_p, nu, ground, mie_p);
         sc.x = sc.x - trans.x * sc_p.x;
         sc.y = sc.y - trans.y * sc_p.y;
         sc.z = sc.z - trans.z * sc_p.z;
@@ -120,19 +125,21 @@ __host__ __device__ inline size_t sky_slice_bytes(const DevParams& P) {
 }
 constexpr size_t kSkySliceMaxBytes = 65536;
 
+template <bool H>
 __device__ void build_sky_slice(const DevParams& P, const float4* __restrict__ T,
                                 const float4* __restrict__ S, float r, float mu_s,
                                 float4* __restrict__ lds) {
     const int NU = P.s_nu, MU = P.s_mu, W = NU * P.s_mus, TMU = P.t_mu;
     const Taps tz = linear_taps<float>(scattering_u_r<float>(P, r), P.s_r);
     const float u_mu_s = scattering_u_mu_s<float>(P, mu_s);
+    constexpr bool h = H;
     for (int e = threadIdx.x; e < (NU + 1) * MU; e += blockDim.x) {
         const int k = e / MU, y = e - k * MU;
         const Taps tx = linear_taps<float>((float(k) + u_mu_s) / float(NU), W);
-        const float4* p0 = S + (size_t(tz.i0) * MU + y) * W;
-        const float4* p1 = S + (size_t(tz.i1) * MU + y) * W;
-        lds[e] = lerp4(lerp4(ldg4(p0 + tx.i0), ldg4(p0 + tx.i1), tx.w),
-                       lerp4(ldg4(p1 + tx.i0), ldg4(p1 + tx.i1), tx.w), tz.w);
+        const size_t p0 = (size_t(tz.i0) * MU + y) * W;
+        const size_t p1 = (size_t(tz.i1) * MU + y) * W;
+        lds[e] = lerp4(lerp4(ld_tex(S, p0 + tx.i0, h), ld_tex(S, p0 + tx.i1, h), tx.w),
+                       lerp4(ld_tex(S, p1 + tx.i0, h), ld_tex(S, p1 + tx.i1, h), tx.w), tz.w);
     }
     float u, v;
     transmittance_uv<float>(P, r, 1.0f, u, v);  // v depends on r only
@@ -183,6 +190,7 @@ struct DrawArgs {
 
 // render_sky.frag:24-35 for one pixel; fullscreen.vert:5-8 interpolates
 // screen_coords to ((px+0.5)/w, (py+0.5)/h) at pixel centres.
+template <bool H>
 __global__ __launch_bounds__(256) void k_render_sky(DevParams P, const float4* __restrict__ T,
                                                     const float4* __restrict__ S, DrawArgs D,
                                                     const float* __restrict__ depth, int w, int h,
@@ -206,7 +214,7 @@ __global__ __launch_bounds__(256) void k_render_sky(DevParams P, const float4* _
         wp[row] = D.ivp[0][row] * n0 + D.ivp[1][row] * n1 + D.ivp[2][row] * dz + D.ivp[3][row];
     float3 world = make_float3(wp[0] / wp[3] * 1e-3f, wp[1] / wp[3] * 1e-3f, wp[2] / wp[3] * 1e-3f);
     float3 rad, tr;
-    sky_radiance_to_point(P, T, S, make_float3(D.cam[0], D.cam[1], D.cam[2]), view, world,
+    sky_radiance_to_point<H>(P, T, S, make_float3(D.cam[0], D.cam[1], D.cam[2]), view, world,
                           make_float3(D.sun[0], D.sun[1], D.sun[2]), rad, tr);
     if (composite) {
         // Dual-source blend dst = src0 + dst * src1 (src/render.rs:124-136);
@@ -220,6 +228,7 @@ __global__ __launch_bounds__(256) void k_render_sky(DevParams P, const float4* _
     if (trans_out) trans_out[idx] = make_float4(tr.x, tr.y, tr.z, 1.0f);
 }
 
+template <bool H>
 __global__ __launch_bounds__(256) void k_render_rays(DevParams P, const float4* __restrict__ T,
                                                      const float4* __restrict__ S,
                                                      const float* __restrict__ cams, int cstride,
@@ -237,9 +246,9 @@ __global__ __launch_bounds__(256) void k_render_rays(DevParams P, const float4* 
     float3 rad, tr;
     if (points) {
         float3 pt = make_float3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
-        sky_radiance_to_point(P, T, S, cam, view, pt, sun, rad, tr);
+        sky_radiance_to_point<H>(P, T, S, cam, view, pt, sun, rad, tr);
     } else {
-        sky_radiance(P, T, S, cam, view, sun, rad, tr);
+        sky_radiance<H>(P, T, S, cam, view, sun, rad, tr);
     }
     rad_out[i] = make_float4(rad.x, rad.y, rad.z, 0.0f);
     if (trans_out) trans_out[i] = make_float4(tr.x, tr.y, tr.z, 1.0f);
@@ -282,6 +291,7 @@ constexpr int kFramePixelsPerThread = 16;
 // Pixels [pix0, pix0 + 256*kFramePixelsPerThread) of one synthetic frame
 // (camera at altitude alt on the z axis, sun at zenith angle zs in the x-z
 // plane).  `slice`: the block's LDS holds room for the frame's LUT slice.
+template <bool H>
 __device__ void render_frame_block(const DevParams& P, const float4* __restrict__ T,
                                    const float4* __restrict__ S, float alt, float zs,
                                    uint32_t frame, uint32_t pixels, uint64_t seed, uint32_t pix0,
@@ -293,7 +303,7 @@ __device__ void render_frame_block(const DevParams& P, const float4* __restrict_
     const float r = sqrtf(dot3(cam, cam));
     const bool use_slice = slice && r <= P.top;  // block-uniform
     if (use_slice) {
-        build_sky_slice(P, T, S, r, dot3(cam, sun) / r, s_slice);
+        build_sky_slice<H>(P, T, S, r, dot3(cam, sun) / r, s_slice);
         __syncthreads();
     }
     for (int j = 0; j < kFramePixelsPerThread; ++j) {
@@ -306,13 +316,14 @@ __device__ void render_frame_block(const DevParams& P, const float4* __restrict_
         if (use_slice)
             sky_radiance_slice(P, s_slice, r, cam, view, sun, rad, tr);
         else
-            sky_radiance(P, T, S, cam, view, sun, rad, tr);
+            sky_radiance<H>(P, T, S, cam, view, sun, rad, tr);
         const size_t o = size_t(frame) * pixels + p;  // frame-major output
         rad_out[o] = make_float4(rad.x, rad.y, rad.z, 0.0f);
         if (trans_out) trans_out[o] = make_float4(tr.x, tr.y, tr.z, 1.0f);
     }
 }
 
+template <bool H>
 __global__ __launch_bounds__(256) void k_render_frames(DevParams P, const float4* __restrict__ T,
                                                        const float4* __restrict__ S,
                                                        const float* __restrict__ altitudes,
@@ -321,7 +332,7 @@ __global__ __launch_bounds__(256) void k_render_frames(DevParams P, const float4
                                                        int slice, float4* __restrict__ rad_out,
                                                        float4* __restrict__ trans_out) {
     const uint32_t f = blockIdx.y;
-    render_frame_block(P, T, S, altitudes[f], zeniths[f], f, pixels, seed,
+    render_frame_block<H>(P, T, S, altitudes[f], zeniths[f], f, pixels, seed,
                        blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0, rad_out,
                        trans_out);
 }
@@ -351,11 +362,12 @@ __global__ __launch_bounds__(256) void k_sun_sky_irradiance(
 
 // One synthetic frame with scalar camera altitude / sun zenith (the serving
 // loop's render request; same rays as frame 0 of k_render_frames).
+template <bool H>
 __global__ __launch_bounds__(256) void k_render_frame1(DevParams P, const float4* __restrict__ T,
                                                        const float4* __restrict__ S, float alt,
                                                        float zs, uint32_t pixels, uint64_t seed,
                                                        int slice, float4* __restrict__ rad_out) {
-    render_frame_block(P, T, S, alt, zs, 0u, pixels, seed,
+    render_frame_block<H>(P, T, S, alt, zs, 0u, pixels, seed,
                        blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0, rad_out,
                        nullptr);
 }
@@ -375,8 +387,12 @@ hipError_t launch_render_frame(const DevParams& P, const float4* T, const float4
                                hipStream_t s) {
     if (pixels == 0) return hipSuccess;
     const size_t lds = frame_lds(P);
-    k_render_frame1<<<frame_blocks(pixels), 256, lds, s>>>(P, T, S, alt, zs, pixels, seed,
-                                                           lds ? 1 : 0, rad);
+    if (P.flags & kFlagF16Storage)
+        k_render_frame1<true><<<frame_blocks(pixels), 256, lds, s>>>(P, T, S, alt, zs, pixels,
+                                                                     seed, lds ? 1 : 0, rad);
+    else
+        k_render_frame1<false><<<frame_blocks(pixels), 256, lds, s>>>(P, T, S, alt, zs, pixels,
+                                                                      seed, lds ? 1 : 0, rad);
     return hipGetLastError();
 }
 
@@ -429,10 +445,15 @@ fb_status fb_render_sky(fb_builder* b, fb_stream stream, const fb_atmosphere* ac
     memcpy(D.cam, draw->camera_position, sizeof(D.cam));
     memcpy(D.sun, draw->sun_direction, sizeof(D.sun));
     dim3 grid((width + 15) / 16, (height + 15) / 16);
-    fbk::k_render_sky<<<grid, 256, 0, s>>>(a->dev, a->T.p, a->S.p, D, depth, int(width),
-                                            int(height), reinterpret_cast<float4*>(color),
-                                            reinterpret_cast<float4*>(transmittance),
-                                            int(flags & FB_RENDER_COMPOSITE));
+    auto* c4 = reinterpret_cast<float4*>(color);
+    auto* t4 = reinterpret_cast<float4*>(transmittance);
+    const int comp = int(flags & FB_RENDER_COMPOSITE);
+    if (a->dev.flags & fbk::kFlagF16Storage)
+        fbk::k_render_sky<true><<<grid, 256, 0, s>>>(a->dev, a->T.p, a->S.p, D, depth, int(width),
+                                                      int(height), c4, t4, comp);
+    else
+        fbk::k_render_sky<false><<<grid, 256, 0, s>>>(a->dev, a->T.p, a->S.p, D, depth,
+                                                       int(width), int(height), c4, t4, comp);
     return end_use(a, s);
 }
 
@@ -450,9 +471,17 @@ fb_status fb_render_sky_rays(fb_builder* b, fb_stream stream, const fb_atmospher
     fb_atmosphere* a = const_cast<fb_atmosphere*>(ac);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if ((st = begin_use(a, s)) != FB_OK) return st;
-    fbk::k_render_rays<<<unsigned((n + 255) / 256), 256, 0, s>>>(
-        a->dev, a->T.p, a->S.p, cameras, int(camera_stride), views, points, suns, int(sun_stride),
-        n, reinterpret_cast<float4*>(radiance), reinterpret_cast<float4*>(transmittance));
+    const unsigned blocks = unsigned((n + 255) / 256);
+    auto* r4 = reinterpret_cast<float4*>(radiance);
+    auto* t4 = reinterpret_cast<float4*>(transmittance);
+    if (a->dev.flags & fbk::kFlagF16Storage)
+        fbk::k_render_rays<true><<<blocks, 256, 0, s>>>(a->dev, a->T.p, a->S.p, cameras,
+                                                         int(camera_stride), views, points, suns,
+                                                         int(sun_stride), n, r4, t4);
+    else
+        fbk::k_render_rays<false><<<blocks, 256, 0, s>>>(a->dev, a->T.p, a->S.p, cameras,
+                                                          int(camera_stride), views, points, suns,
+                                                          int(sun_stride), n, r4, t4);
     return end_use(a, s);
 }
 
@@ -472,10 +501,16 @@ fb_status fb_render_sky_frames(fb_builder* b, fb_stream stream, const fb_atmosph
     if ((st = begin_use(a, s)) != FB_OK) return st;
     dim3 grid(fbk::frame_blocks(pixels), n_frames);
     const size_t lds = fbk::frame_lds(a->dev);
-    fbk::k_render_frames<<<grid, 256, lds, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
-                                                 sun_zenith_rad, pixels, seed, lds ? 1 : 0,
-                                                 reinterpret_cast<float4*>(radiance),
-                                                 reinterpret_cast<float4*>(transmittance));
+    auto* r4 = reinterpret_cast<float4*>(radiance);
+    auto* t4 = reinterpret_cast<float4*>(transmittance);
+    if (a->dev.flags & fbk::kFlagF16Storage)
+        fbk::k_render_frames<true><<<grid, 256, lds, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
+                                                           sun_zenith_rad, pixels, seed,
+                                                           lds ? 1 : 0, r4, t4);
+    else
+        fbk::k_render_frames<false><<<grid, 256, lds, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
+                                                            sun_zenith_rad, pixels, seed,
+                                                            lds ? 1 : 0, r4, t4);
     return end_use(a, s);
 }
 

@@ -147,8 +147,14 @@ fbk::DevParams make_dev_params(const fb_parameters& p) {
 }
 
 // ----------------------------------------------------------- allocation
-static fb_status alloc_buf(fb_builder* b, size_t texels, Buf* out) {
-    out->bytes = texels * sizeof(float4);
+static_assert(fbk::kFlagF16Storage == int(FB_FLAG_F16_STORAGE), "storage flag mirror");
+// Bytes per texel of the scattering LUT and the 3D scratch LUTs.
+static size_t s_texel_bytes(const fb_parameters& p) {
+    return (p.flags & FB_FLAG_F16_STORAGE) ? 8 : sizeof(float4);
+}
+static fb_status alloc_buf(fb_builder* b, size_t texels, Buf* out,
+                           size_t texel_bytes = sizeof(float4)) {
+    out->bytes = texels * texel_bytes;
     void* p = nullptr;
     fb_status st = b->pool->alloc(out->bytes, &p);
     out->p = static_cast<float4*>(p);
@@ -195,12 +201,13 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
     fb_status st = FB_OK;
     if (st == FB_OK) st = alloc_buf(b, s.nt, &a->T);
     if (st == FB_OK) st = alloc_buf(b, s.ni, &a->E);
-    if (st == FB_OK) st = alloc_buf(b, s.ns, &a->S);
+    const size_t tb = s_texel_bytes(p);
+    if (st == FB_OK) st = alloc_buf(b, s.ns, &a->S, tb);
     if (st == FB_OK) st = alloc_buf(b, s.ni, &pa->dE);
-    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dR);
-    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dM);
-    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dMS);
-    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->J);
+    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dR, tb);
+    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dM, tb);
+    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->dMS, tb);
+    if (st == FB_OK) st = alloc_buf(b, s.ns, &pa->J, tb);
     if (st == FB_OK) st = alloc_buf(b, s.nmr, &pa->MR);
     if (st == FB_OK) st = alloc_buf(b, s.nt, &pa->OD);
     if (st == FB_OK) st = alloc_buf(b, s.ndg, &pa->DG);
@@ -434,7 +441,7 @@ fb_status fb_parameters_validate(const fb_parameters* p) {
                 set_error("coefficients must be finite and non-negative");
                 return FB_EINVAL;
             }
-    if (p->flags & ~FB_FLAG_PRECISE) {
+    if (p->flags & ~(FB_FLAG_PRECISE | FB_FLAG_F16_STORAGE)) {
         set_error("unknown flags 0x%x", p->flags);
         return FB_EINVAL;
     }
@@ -463,8 +470,9 @@ void fb_parameters_extents(const fb_parameters* p, uint32_t transmittance[2],
 size_t fb_parameters_footprint(const fb_parameters* p, size_t* resident, size_t* scratch) {
     if (!p) return 0;
     Sizes s = sizes_of(*p);
-    size_t res = (s.nt + s.ni + s.ns) * sizeof(float4);
-    size_t scr = (s.nt + s.ni + 4 * s.ns + s.nmr + s.ndg + s.nsr) * sizeof(float4);
+    const size_t tb = s_texel_bytes(*p);
+    size_t res = (s.nt + s.ni) * sizeof(float4) + s.ns * tb;
+    size_t scr = (s.nt + s.ni + s.nmr + s.ndg + s.nsr) * sizeof(float4) + 4 * s.ns * tb;
     if (resident) *resident = res;
     if (scratch) *scratch = scr;
     return res + scr;
@@ -677,8 +685,8 @@ void fb_pending_destroy(fb_pending_atmosphere* pa) {
 static fb_status view_of(const fb_parameters& p, fb_lut which, const Buf& buf,
                          fb_lut_view* out) {
     out->device_ptr = buf.p;
-    out->texel_bytes = sizeof(float4);
     out->bytes = buf.bytes;
+    out->texel_bytes = uint32_t(sizeof(float4));
     switch (which) {
         case FB_LUT_TRANSMITTANCE:
             out->width = p.transmittance_mu_size;
@@ -695,6 +703,7 @@ static fb_status view_of(const fb_parameters& p, fb_lut which, const Buf& buf,
             out->width = p.scattering_nu_size * p.scattering_mu_s_size;
             out->height = p.scattering_mu_size;
             out->depth = p.scattering_r_size;
+            out->texel_bytes = uint32_t(s_texel_bytes(p));
             break;
     }
     return FB_OK;
@@ -748,6 +757,11 @@ static uint16_t f32_to_f16(float f) {
     memcpy(&u, &h, 2);
     return u;
 }
+static float f16_to_f32(uint16_t u) {
+    _Float16 h;
+    memcpy(&h, &u, 2);
+    return static_cast<float>(h);
+}
 
 fb_status fb_atmosphere_download(const fb_atmosphere* a, fb_lut which, uint32_t format,
                                  void* host, size_t bytes) {
@@ -758,8 +772,8 @@ fb_status fb_atmosphere_download(const fb_atmosphere* a, fb_lut which, uint32_t 
     fb_lut_view v;
     fb_status st = fb_atmosphere_lut(a, which, &v);
     if (st != FB_OK) return st;
-    size_t texels = v.bytes / sizeof(float4);
-    size_t need = format == FB_FORMAT_F16 ? texels * 8 : v.bytes;
+    size_t texels = v.bytes / v.texel_bytes;
+    size_t need = texels * (format == FB_FORMAT_F16 ? 8 : sizeof(float4));
     if (format != FB_FORMAT_F32 && format != FB_FORMAT_F16) {
         set_error("unknown format %u", format);
         return FB_EINVAL;
@@ -770,8 +784,15 @@ fb_status fb_atmosphere_download(const fb_atmosphere* a, fb_lut which, uint32_t 
     }
     FB_HIP_TRY(hipSetDevice(a->builder->device));
     FB_HIP_TRY(hipEventSynchronize(a->ready));
-    if (format == FB_FORMAT_F32) {
+    const bool stored_f16 = v.texel_bytes == 8;
+    if ((format == FB_FORMAT_F16) == stored_f16) {
+        // Stored in the requested format: a plain copy.
         FB_HIP_TRY(hipMemcpy(host, v.device_ptr, v.bytes, hipMemcpyDeviceToHost));
+    } else if (stored_f16) {
+        std::vector<uint16_t> tmp(texels * 4);
+        FB_HIP_TRY(hipMemcpy(tmp.data(), v.device_ptr, v.bytes, hipMemcpyDeviceToHost));
+        float* dst = static_cast<float*>(host);
+        for (size_t i = 0; i < texels * 4; ++i) dst[i] = f16_to_f32(tmp[i]);
     } else {
         std::vector<float> tmp(texels * 4);
         FB_HIP_TRY(hipMemcpy(tmp.data(), v.device_ptr, v.bytes, hipMemcpyDeviceToHost));

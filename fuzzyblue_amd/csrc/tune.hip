@@ -16,7 +16,7 @@
 namespace fbrt {
 
 TuneKey tune_key(const fb_parameters& p) {
-    return TuneKey{p.flags & FB_FLAG_PRECISE, p.transmittance_mu_size, p.transmittance_r_size,
+    return TuneKey{p.flags & (FB_FLAG_PRECISE | FB_FLAG_F16_STORAGE), p.transmittance_mu_size, p.transmittance_r_size,
                    p.scattering_r_size,       p.scattering_mu_size,    p.scattering_mu_s_size,
                    p.scattering_nu_size,      p.irradiance_mu_s_size,  p.irradiance_r_size};
 }

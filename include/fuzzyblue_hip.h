@@ -15,8 +15,9 @@
  *    passed as fb_stream; NULL means the device's default stream), mirroring
  *    the reference's record-only Atmosphere::build (src/precompute.rs:1077).
  *    No call synchronises the device except the download / wait helpers.
- *  - All LUTs live in HBM as row-major float4 (RGBA32F) arrays: x fastest,
- *    then y, then z.  The scattering LUT is the reference's 4D table packed
+ *  - All LUTs live in HBM as row-major float4 (RGBA32F) arrays (RGBA
+ *    binary16 for the scattering-sized LUTs with FB_FLAG_F16_STORAGE): x
+ *    fastest, then y, then z.  The scattering LUT is the reference's 4D table packed
  *    as a 3D extent (NU*MUS) x MU x R with x = nu*MUS + mu_s
  *    (src/precompute.rs:786-791, shaders/scattering.h:120-131), identical to
  *    the linear readback layout of examples/dump.rs:175-193.
@@ -79,6 +80,15 @@ typedef struct fb_density_profile {
 /* fb_parameters.flags */
 #define FB_FLAG_PRECISE 0x1u /* FP64 ray-sphere geometry and horizon
                                 parameterisation, compensated sums */
+/* The reference's storage formats (src/precompute.rs:1215): the scattering
+ * LUT and the four 3D scratch LUTs (delta Rayleigh, delta Mie, delta
+ * multiple scattering, scattering density) are RGBA binary16, 8 bytes per
+ * texel, every store rounded to nearest even as the reference's
+ * R16G16B16A16_SFLOAT image stores are; arithmetic stays FP32 (FP64
+ * geometry with FB_FLAG_PRECISE).  Transmittance and the irradiance LUTs
+ * stay RGBA FP32, as in the reference (:1170, :1191).  Views of the
+ * binary16 LUTs report texel_bytes = 8. */
+#define FB_FLAG_F16_STORAGE 0x2u
 
 typedef struct fb_parameters {
     uint32_t order; /* number of light bounces; 1 = single scattering only */
@@ -275,11 +285,12 @@ typedef enum fb_lut {
 } fb_lut;
 
 typedef struct fb_lut_view {
-    void *device_ptr;  /* float4 texels, row-major x,y,z */
+    void *device_ptr;  /* RGBA texels, row-major x,y,z */
     uint32_t width;
     uint32_t height;
     uint32_t depth;    /* 1 for 2D LUTs */
-    uint32_t texel_bytes; /* 16 */
+    uint32_t texel_bytes; /* 16 (float32 x4), or 8 (binary16 x4) for the
+                             scattering-sized LUTs with FB_FLAG_F16_STORAGE */
     size_t bytes;
 } fb_lut_view;
 
@@ -297,7 +308,9 @@ fb_status fb_atmosphere_parameters(const fb_atmosphere *a, fb_parameters *out);
 /* Copy a LUT to host memory in the examples/dump.rs:110-126 linear layout.
  * format 0: float32 x4 (16 B/texel); format 1: float16 x4 (8 B/texel, the
  * reference's R16G16B16A16_SFLOAT scattering format, src/precompute.rs:1215).
- * Synchronises with the atmosphere's completion event. */
+ * A LUT stored in the other format is converted (binary16 widens exactly,
+ * float32 rounds to nearest even).  Synchronises with the atmosphere's
+ * completion event. */
 #define FB_FORMAT_F32 0u
 #define FB_FORMAT_F16 1u
 fb_status fb_atmosphere_download(const fb_atmosphere *a, fb_lut which,

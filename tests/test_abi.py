@@ -64,6 +64,23 @@ def test_footprint():
     assert tot == res.value + scr.value
 
 
+def test_footprint_f16_storage():
+    """FB_FLAG_F16_STORAGE: the scattering LUT and the four scattering-sized
+    scratch LUTs take 8 bytes per texel (R16G16B16A16_SFLOAT,
+    src/precompute.rs:1215); everything else is unchanged."""
+    lib = load()
+    full = [C.c_size_t(), C.c_size_t()]
+    half = [C.c_size_t(), C.c_size_t()]
+    p = Parameters()
+    lib.fb_parameters_footprint(C.byref(p.to_c()), C.byref(full[0]), C.byref(full[1]))
+    h = p.replace(flags=abi.FB_FLAG_F16_STORAGE | abi.FB_FLAG_PRECISE)
+    check(lib.fb_parameters_validate(C.byref(h.to_c())))
+    lib.fb_parameters_footprint(C.byref(h.to_c()), C.byref(half[0]), C.byref(half[1]))
+    ns = 256 * 128 * 32
+    assert full[0].value - half[0].value == ns * 8
+    assert full[1].value - half[1].value == 4 * ns * 8
+
+
 @pytest.mark.parametrize(
     "field,value",
     [
@@ -78,6 +95,7 @@ def test_footprint():
         ("mu_s_min", 0.1),
         ("sun_angular_radius", 0.2),
         ("flags", 0x80),
+        ("flags", 0x4),
         ("scattering_nu_size", 1000),
     ],
 )

@@ -412,6 +412,45 @@ def test_fp32_restatement_tracks_fp64_away_from_horizon():
     assert np.quantile(rel, 0.9) < 1e-3
 
 
+def f16(x):
+    return np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
+
+
+def test_fp16_storage_rounds_every_3d_store():
+    """fp16 storage (the reference's R16G16B16A16_SFLOAT images,
+    src/precompute.rs:1215): each pass computes in FP32 and rounds what it
+    stores into a 3D texture to binary16 (numpy's IEEE round to nearest
+    even); 2D textures stay FP32.  On the same inputs, every fp16-mode
+    output equals the FP32-mode output rounded."""
+    p = SMOKE
+    T = O32.pass_transmittance(p)
+    dR, dM, S1 = O32.pass_single_scattering(p, T)
+    hR, hM, hS1 = O32.pass_single_scattering(p, T, fp16=True)
+    for a, b in ((hR, dR), (hM, dM), (hS1, S1)):
+        assert np.array_equal(a, f16(b))
+    dE = O32.pass_direct_irradiance(p, T)
+    zeros = np.zeros_like(hR)
+    J = O32.pass_density(p, 2, T, hR, hM, zeros, dE)
+    hJ = O32.pass_density(p, 2, T, hR, hM, zeros, dE, fp16=True)
+    assert np.array_equal(hJ, f16(J))
+    dMS, S2 = O32.pass_multiple(p, T, hJ, hS1)
+    hMS, hS2 = O32.pass_multiple(p, T, hJ, hS1, fp16=True)
+    assert np.array_equal(hMS, f16(dMS))
+    assert np.array_equal(hS2, f16(S2))  # S + ms / P_R in FP32, then stored
+    # end to end: every scattering texel is a binary16 value.  Against FP32
+    # storage the typical difference is the binary16 rounding; the density
+    # LUT's small values (~1e-8 on the night side) fall below the binary16
+    # subnormal range, so there the multiple scattering is lost, as in the
+    # reference's 16F images: a floor of 1e-3 x max bounds those texels.
+    h = O32.precompute(p, fp16=True)
+    f = O32.precompute(p)
+    assert np.array_equal(h["S"], f16(h["S"]))
+    assert not np.array_equal(h["E"], f16(h["E"]))  # irradiance stays FP32
+    rel = np.abs(h["S"][..., :3] - f["S"][..., :3]) / np.maximum(
+        np.abs(f["S"][..., :3]), 1e-3 * np.abs(f["S"]).max())
+    assert np.median(rel) < 1e-3 and rel.max() < 0.05
+
+
 # ------------------------------------------------------------------ render
 def test_sky_radiance_ray_missing_atmosphere():
     """render_sky.h:64-69: a viewer in space whose ray misses the atmosphere
