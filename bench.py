@@ -474,6 +474,16 @@ def main():
     pa32, ms32 = measure_throughput(b, stream, p.replace(flags=0), args.steps, args.warmup)
     pa32.close()
 
+    # The reference's storage formats (binary16 scattering and 3D scratch),
+    # precision-mode arithmetic, default block shapes.
+    log("[bench] binary16-storage throughput")
+    p16 = p.replace(flags=abi.FB_FLAG_PRECISE | abi.FB_FLAG_F16_STORAGE)
+    pa16, ms16 = measure_throughput(b, stream, p16, args.steps, args.warmup)
+    f16_storage = {"lut_sets_s": args.steps / (ms16 * 1e-3),
+                   "max_rel_err_vs_fp64": max_err_vs_fixture(pa16.atmosphere()),
+                   "scattering_lut_bytes": pa16.atmosphere().lut("scattering").bytes}
+    pa16.close()
+
     sky = None
     if not args.no_render:
         log("[bench] render")
@@ -536,6 +546,7 @@ def main():
         "tuning": {"precise": tuning, "fp32": tuning32},
         "max_rel_err_vs_fp64": err,
         "fp32_mode_lut_sets_s": args.steps / (ms32 * 1e-3),
+        "f16_storage": f16_storage,
         "sky": sky,
         "batch_mixed256": batch,
         "serving": serving,

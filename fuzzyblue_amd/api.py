@@ -337,11 +337,21 @@ class Renderer:
     def rays(self, stream, atmosphere: Atmosphere, cameras: DeviceBuffer, camera_stride: int,
              views: DeviceBuffer, suns: DeviceBuffer, sun_stride: int, n: int,
              radiance: DeviceBuffer, transmittance: Optional[DeviceBuffer] = None,
-             points: Optional[DeviceBuffer] = None):
-        check(load().fb_render_sky_rays(
-            self.builder.handle, _stream_handle(stream), atmosphere.handle, cameras.ptr,
-            camera_stride, views.ptr, points.ptr if points is not None else None, suns.ptr,
-            sun_stride, n, radiance.ptr, transmittance.ptr if transmittance is not None else None))
+             points: Optional[DeviceBuffer] = None,
+             shadow_lengths: Optional[DeviceBuffer] = None):
+        """Per-ray lookups; shadow_lengths (float per ray, km) adds light
+        shafts (fb_render_sky_rays_shadowed)."""
+        pts = points.ptr if points is not None else None
+        tr = transmittance.ptr if transmittance is not None else None
+        if shadow_lengths is None:
+            check(load().fb_render_sky_rays(
+                self.builder.handle, _stream_handle(stream), atmosphere.handle, cameras.ptr,
+                camera_stride, views.ptr, pts, suns.ptr, sun_stride, n, radiance.ptr, tr))
+        else:
+            check(load().fb_render_sky_rays_shadowed(
+                self.builder.handle, _stream_handle(stream), atmosphere.handle, cameras.ptr,
+                camera_stride, views.ptr, pts, suns.ptr, sun_stride, shadow_lengths.ptr, n,
+                radiance.ptr, tr))
 
     def frames(self, stream, atmosphere: Atmosphere, altitudes: DeviceBuffer,
                zeniths: DeviceBuffer, n_frames: int, pixels: int, seed: int,

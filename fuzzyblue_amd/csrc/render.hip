@@ -36,11 +36,14 @@ __device__ __forceinline__ float4 combined_scattering(const DevParams& P,
     return c;
 }
 
-// render_sky.h:45-109
-template <bool H>
+// render_sky.h:45-109.  SHADOW: take the light-shaft term the reference
+// keeps commented out (render_sky.h:81-106) for shadow_length > 0; a
+// shadow_length of 0 is the reference's path.
+template <bool H, bool SHADOW = false>
 __device__ void sky_radiance(const DevParams& P, const float4* __restrict__ T,
                              const float4* __restrict__ S, float3 camera, float3 view,
-                             float3 sun, float3& rad, float3& trans) {
+                             float3 sun, float3& rad, float3& trans,
+                             float shadow_length = 0.0f) {
     float r = sqrtf(dot3(camera, camera));
     float rmu = dot3(camera, view);
     float dtop = -rmu - sqrtf(rmu * rmu - r * r + P.top * P.top);
@@ -60,16 +63,32 @@ __device__ void sky_radiance(const DevParams& P, const float4* __restrict__ T,
     bool ground = ray_intersects_ground<float>(P, r, mu);
     trans = ground ? make_float3(0.0f, 0.0f, 0.0f) : transmittance_to_top<float>(P, T, r, mu);
     float3 mie;
-    float4 sc = combined_scattering<H>(P, S, r, mu, mu_s, nu, ground, mie);
+    float4 sc;
+    if (!SHADOW || shadow_length == 0.0f) {
+        sc = combined_scattering<H>(P, S, r, mu, mu_s, nu, ground, mie);
+    } else {
+        // Eq. (18): the scattering seen from x + l v, attenuated by
+        // T(x, x + l v) (render_sky.h:86-105).
+        const float d = shadow_length;
+        const float r_p = clamp_radius<float>(P, sqrtf(d * d + 2.0f * r * mu * d + r * r));
+        const float mu_p = (r * mu + d) / r_p;
+        const float mu_s_p = (r * mu_s + d * nu) / r_p;
+        sc = combined_scattering<H>(P, S, r_p, mu_p, mu_s_p, nu, ground, mie);
+        const float3 st = get_transmittance<float>(P, T, r, mu, shadow_length, ground);
+        sc = make_float4(sc.x * st.x, sc.y * st.y, sc.z * st.z, sc.w);
+        mie = make_float3(mie.x * st.x, mie.y * st.y, mie.z * st.z);
+    }
     float pr = phase_rayleigh(nu), pm = phase_mie(P.g, nu);
     rad = make_float3(sc.x * pr + mie.x * pm, sc.y * pr + mie.y * pm, sc.z * pr + mie.z * pm);
 }
 
-// render_sky.h:111-191
-template <bool H>
+// render_sky.h:111-191, with the commented-out light-shaft term
+// (:158-160, :172-177) when SHADOW and shadow_length > 0.
+template <bool H, bool SHADOW = false>
 __device__ void sky_radiance_to_point(const DevParams& P, const float4* __restrict__ T,
                                       const float4* __restrict__ S, float3 camera, float3 view,
-                                      float3 point, float3 sun, float3& rad, float3& trans) {
+                                      float3 point, float3 sun, float3& rad, float3& trans,
+                                      float shadow_length = 0.0f) {
     float r = sqrtf(dot3(camera, camera));
     float rmu = dot3(camera, view);
     float dtop = -rmu - sqrtf(rmu * rmu - r * r + P.top * P.top);
@@ -93,15 +112,21 @@ __device__ void sky_radiance_to_point(const DevParams& P, const float4* __restri
     float3 mie;
     float4 sc = combined_scattering<H>(P, S, r, mu, mu_s, nu, ground, mie);
     if (!isinf(d)) {
+        // Eq. (17): the second lookup at d - l, attenuated by T(x, x_{d-l}).
+        float3 st = trans;
+        if (SHADOW && shadow_length > 0.0f) {
+            d = fmaxf(d - shadow_length, 0.0f);
+            st = get_transmittance<float>(P, T, r, mu, d, ground);
+        }
         float r_p = clamp_radius<float>(P, sqrtf(d * d + 2.0f * r * mu * d + r * r));
         float mu_p = (r * mu + d) / r_p;
         float mu_s_p = (r * mu_s + d * nu) / r_p;
         float3 mie_p;
         float4 sc_p = combined_scattering<H>(P, S, r_p, mu_p, mu_s_p, nu, ground, mie_p);
-        sc.x = sc.x - trans.x * sc_p.x;
-        sc.y = sc.y - trans.y * sc_p.y;
-        sc.z = sc.z - trans.z * sc_p.z;
-        float mr = mie.x - trans.x * mie_p.x;
+        sc.x = sc.x - st.x * sc_p.x;
+        sc.y = sc.y - st.y * sc_p.y;
+        sc.z = sc.z - st.z * sc_p.z;
+        float mr = mie.x - st.x * mie_p.x;
         mie = extrapolated_single_mie(P, make_float4(sc.x, sc.y, sc.z, mr));
         float h = smoothstep_g<float>(0.0f, 0.01f, mu_s);
         mie = make_float3(mie.x * h, mie.y * h, mie.z * h);
@@ -235,10 +260,12 @@ __global__ __launch_bounds__(256) void k_render_rays(DevParams P, const float4* 
                                                      const float* __restrict__ views,
                                                      const float* __restrict__ points,
                                                      const float* __restrict__ suns, int sstride,
+                                                     const float* __restrict__ shadow,
                                                      size_t n, float4* __restrict__ rad_out,
                                                      float4* __restrict__ trans_out) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const float l = shadow ? shadow[i] : 0.0f;
     size_t ci = cstride ? i : 0, si = sstride ? i : 0;
     float3 cam = make_float3(cams[3 * ci], cams[3 * ci + 1], cams[3 * ci + 2]);
     float3 sun = make_float3(suns[3 * si], suns[3 * si + 1], suns[3 * si + 2]);
@@ -246,9 +273,9 @@ __global__ __launch_bounds__(256) void k_render_rays(DevParams P, const float4* 
     float3 rad, tr;
     if (points) {
         float3 pt = make_float3(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
-        sky_radiance_to_point<H>(P, T, S, cam, view, pt, sun, rad, tr);
+        sky_radiance_to_point<H, true>(P, T, S, cam, view, pt, sun, rad, tr, l);
     } else {
-        sky_radiance<H>(P, T, S, cam, view, sun, rad, tr);
+        sky_radiance<H, true>(P, T, S, cam, view, sun, rad, tr, l);
     }
     rad_out[i] = make_float4(rad.x, rad.y, rad.z, 0.0f);
     if (trans_out) trans_out[i] = make_float4(tr.x, tr.y, tr.z, 1.0f);
@@ -457,10 +484,12 @@ fb_status fb_render_sky(fb_builder* b, fb_stream stream, const fb_atmosphere* ac
     return end_use(a, s);
 }
 
-fb_status fb_render_sky_rays(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,
-                             const float* cameras, uint32_t camera_stride, const float* views,
-                             const float* points, const float* suns, uint32_t sun_stride,
-                             size_t n, float* radiance, float* transmittance) {
+fb_status fb_render_sky_rays_shadowed(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,
+                                      const float* cameras, uint32_t camera_stride,
+                                      const float* views, const float* points,
+                                      const float* suns, uint32_t sun_stride,
+                                      const float* shadow_lengths, size_t n, float* radiance,
+                                      float* transmittance) {
     fb_status st = check_atm(b, ac);
     if (st != FB_OK) return st;
     if (!cameras || !views || !suns || !radiance || camera_stride > 1 || sun_stride > 1) {
@@ -477,12 +506,22 @@ fb_status fb_render_sky_rays(fb_builder* b, fb_stream stream, const fb_atmospher
     if (a->dev.flags & fbk::kFlagF16Storage)
         fbk::k_render_rays<true><<<blocks, 256, 0, s>>>(a->dev, a->T.p, a->S.p, cameras,
                                                          int(camera_stride), views, points, suns,
-                                                         int(sun_stride), n, r4, t4);
+                                                         int(sun_stride), shadow_lengths, n, r4,
+                                                         t4);
     else
         fbk::k_render_rays<false><<<blocks, 256, 0, s>>>(a->dev, a->T.p, a->S.p, cameras,
                                                           int(camera_stride), views, points, suns,
-                                                          int(sun_stride), n, r4, t4);
+                                                          int(sun_stride), shadow_lengths, n, r4,
+                                                          t4);
     return end_use(a, s);
+}
+
+fb_status fb_render_sky_rays(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,
+                             const float* cameras, uint32_t camera_stride, const float* views,
+                             const float* points, const float* suns, uint32_t sun_stride,
+                             size_t n, float* radiance, float* transmittance) {
+    return fb_render_sky_rays_shadowed(b, stream, ac, cameras, camera_stride, views, points,
+                                       suns, sun_stride, nullptr, n, radiance, transmittance);
 }
 
 fb_status fb_render_sky_frames(fb_builder* b, fb_stream stream, const fb_atmosphere* ac,

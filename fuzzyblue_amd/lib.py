@@ -89,6 +89,10 @@ def load():
             S,
             [V, V, V, V, C.c_uint32, V, V, V, C.c_uint32, C.c_size_t, V, V],
         ),
+        "fb_render_sky_rays_shadowed": (
+            S,
+            [V, V, V, V, C.c_uint32, V, V, V, C.c_uint32, V, C.c_size_t, V, V],
+        ),
         "fb_render_sky_frames": (
             S,
             [V, V, V, V, V, C.c_uint32, C.c_uint32, C.c_uint64, V, V],

@@ -359,6 +359,23 @@ fb_status fb_render_sky_rays(fb_builder *b, fb_stream stream,
                              uint32_t sun_stride, size_t n, float *radiance,
                              float *transmittance);
 
+/* fb_render_sky_rays with light shafts: the shadow_length term the
+ * reference keeps commented out (shaders/render_sky.h:81-106, :160,
+ * :173-177; Bruneton & Neyret 2008, Eqs. 17-18).  shadow_lengths (device
+ * float, km, one per ray; NULL = all 0) is the length l of the ray's
+ * shadowed part.  GetSkyRadiance omits the scattering of the first l km
+ * from the viewer (the lookup moves to x + l v and is attenuated by
+ * T(x, x + l v)); GetSkyRadianceToPoint omits the last l km before the
+ * point (its second lookup moves to d - l, attenuated by T(x, x_{d-l})); the
+ * transmittance output is unchanged.  l = 0 gives fb_render_sky_rays bit for
+ * bit. */
+fb_status fb_render_sky_rays_shadowed(fb_builder *b, fb_stream stream,
+                                      const fb_atmosphere *a, const float *cameras,
+                                      uint32_t camera_stride, const float *views,
+                                      const float *points, const float *suns,
+                                      uint32_t sun_stride, const float *shadow_lengths,
+                                      size_t n, float *radiance, float *transmittance);
+
 /* Synthetic frames for throughput measurement: frame f has camera
  * (0, 0, bottom + altitude[f]) and sun (sin zs[f], 0, cos zs[f]); pixel i of
  * frame f looks along a direction uniform on the sphere drawn from a

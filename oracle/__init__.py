@@ -97,7 +97,7 @@ class Oracle:
             "oracle_precompute": (C.c_int, [P, C.c_int, V, V, V, V, V]),
             "oracle_sky_rays": (
                 None,
-                [P, V, V, V, C.c_int, V, V, V, C.c_int, C.c_long, V, V],
+                [P, V, V, V, C.c_int, V, V, V, C.c_int, V, C.c_long, V, V],
             ),
             "oracle_render_frame": (None, [P, V, V, V, V, V, V, C.c_int, C.c_int, V, V]),
             "oracle_sun_and_sky_irradiance": (None, [P, V, V, V, V, V, C.c_int, C.c_long, V, V]),
@@ -349,7 +349,7 @@ class Oracle:
             out["orders_E"] = po_e
         return out
 
-    def sky_rays(self, p, T, S, cameras, views, suns, points=None):
+    def sky_rays(self, p, T, S, cameras, views, suns, points=None, shadow=None):
         c = self.cparams(p)
         T, S = self.asarr(T), self.asarr(S)
         cams = self.asarr(np.atleast_2d(cameras))
@@ -357,6 +357,7 @@ class Oracle:
         views = self.asarr(np.atleast_2d(views))
         n = views.shape[0]
         pts = self.asarr(np.atleast_2d(points)) if points is not None else None
+        sh = self.asarr(np.asarray(shadow).reshape(n)) if shadow is not None else None
         rad, trans = self.arr(n, 3), self.arr(n, 3)
         self._fn("oracle_sky_rays")(
             C.byref(c),
@@ -368,6 +369,7 @@ class Oracle:
             _ptr(pts) if pts is not None else None,
             _ptr(suns_),
             1 if suns_.shape[0] > 1 else 0,
+            _ptr(sh) if sh is not None else None,
             n,
             _ptr(rad),
             _ptr(trans),

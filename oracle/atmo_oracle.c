@@ -870,9 +870,11 @@ static void combined_scattering(const atmo_t *a, const real *S, real r, real mu,
     rgb[1] = c4[1];
     rgb[2] = c4[2];
 }
-/* render_sky.h:45-109 */
+/* render_sky.h:45-109; shadow_length > 0 restates the light-shaft branch the
+ * reference keeps commented out (render_sky.h:81-106). */
 static void sky_radiance(const atmo_t *a, const real *T, const real *S, const real cam_in[3],
-                         const real view[3], const real sun[3], real rad[3], real trans[3]) {
+                         const real view[3], const real sun[3], real shadow_length, real rad[3],
+                         real trans[3]) {
     real camera[3] = {cam_in[0], cam_in[1], cam_in[2]};
     real r = sqrt(dot3(camera, camera));
     real rmu = dot3(camera, view);
@@ -896,15 +898,30 @@ static void sky_radiance(const atmo_t *a, const real *T, const real *S, const re
         transmittance_to_top(a, T, r, mu, trans);
     }
     real sc[3], mie[3];
-    combined_scattering(a, S, r, mu, mu_s, nu, ground, sc, mie);
+    if (shadow_length == 0.0) {
+        combined_scattering(a, S, r, mu, mu_s, nu, ground, sc, mie);
+    } else {
+        real d = shadow_length;
+        real r_p = clamp_radius(a, sqrt(d * d + 2.0 * r * mu * d + r * r));
+        real mu_p = (r * mu + d) / r_p;
+        real mu_s_p = (r * mu_s + d * nu) / r_p;
+        combined_scattering(a, S, r_p, mu_p, mu_s_p, nu, ground, sc, mie);
+        real st[3];
+        get_transmittance(a, T, r, mu, shadow_length, ground, st);
+        for (int c = 0; c < 3; ++c) {
+            sc[c] = sc[c] * st[c];
+            mie[c] = mie[c] * st[c];
+        }
+    }
     real pr = phase_rayleigh(nu), pm = phase_mie(a->g, nu);
     for (int c = 0; c < 3; ++c) rad[c] = sc[c] * pr + mie[c] * pm;
 }
-/* render_sky.h:111-191 */
+/* render_sky.h:111-191; shadow_length > 0 restates the commented-out
+ * light-shaft lines (render_sky.h:158-160, :172-177). */
 static void sky_radiance_to_point(const atmo_t *a, const real *T, const real *S,
                                   const real cam_in[3], const real view[3],
-                                  const real point[3], const real sun[3], real rad[3],
-                                  real trans[3]) {
+                                  const real point[3], const real sun[3], real shadow_length,
+                                  real rad[3], real trans[3]) {
     real camera[3] = {cam_in[0], cam_in[1], cam_in[2]};
     real r = sqrt(dot3(camera, camera));
     real rmu = dot3(camera, view);
@@ -928,14 +945,19 @@ static void sky_radiance_to_point(const atmo_t *a, const real *T, const real *S,
     real sc[3], mie[3];
     combined_scattering(a, S, r, mu, mu_s, nu, ground, sc, mie);
     if (!isinf(d)) {
+        real st[3] = {trans[0], trans[1], trans[2]};
+        if (shadow_length > 0.0) {
+            d = fmax(d - shadow_length, 0.0);
+            get_transmittance(a, T, r, mu, d, ground, st);
+        }
         real r_p = clamp_radius(a, sqrt(d * d + 2.0 * r * mu * d + r * r));
         real mu_p = (r * mu + d) / r_p;
         real mu_s_p = (r * mu_s + d * nu) / r_p;
         real sc_p[3], mie_p[3];
         combined_scattering(a, S, r_p, mu_p, mu_s_p, nu, ground, sc_p, mie_p);
         for (int c = 0; c < 3; ++c) {
-            sc[c] = sc[c] - trans[c] * sc_p[c];
-            mie[c] = mie[c] - trans[c] * mie_p[c];
+            sc[c] = sc[c] - st[c] * sc_p[c];
+            mie[c] = mie[c] - st[c] * mie_p[c];
         }
         real s4[4] = {sc[0], sc[1], sc[2], mie[0]};
         extrapolated_single_mie(a, s4, mie);
@@ -1186,22 +1208,24 @@ EXPORT int FN(oracle_precompute)(const fb_parameters *p, int fp16, real *T, real
 }
 
 /* Per-ray sky lookups (render_sky.h:45-191).  point == NULL selects
- * GetSkyRadiance, else GetSkyRadianceToPoint.  Arrays are packed 3-vectors. */
+ * GetSkyRadiance, else GetSkyRadianceToPoint.  Arrays are packed 3-vectors;
+ * shadow (one length per ray, km) may be NULL (no light shafts). */
 EXPORT void FN(oracle_sky_rays)(const fb_parameters *p, const real *T, const real *S,
                                 const real *cameras, int camera_stride, const real *views,
                                 const real *points, const real *suns, int sun_stride,
-                                long n, real *rad, real *trans) {
+                                const real *shadow, long n, real *rad, real *trans) {
     atmo_t a;
     load_atmo(&a, p, 0);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) {
         const real *cam = cameras + 3 * (camera_stride ? i : 0);
         const real *sun = suns + 3 * (sun_stride ? i : 0);
+        const real l = shadow ? shadow[i] : 0.0;
         if (points)
-            sky_radiance_to_point(&a, T, S, cam, views + 3 * i, points + 3 * i, sun,
+            sky_radiance_to_point(&a, T, S, cam, views + 3 * i, points + 3 * i, sun, l,
                                   rad + 3 * i, trans + 3 * i);
         else
-            sky_radiance(&a, T, S, cam, views + 3 * i, sun, rad + 3 * i, trans + 3 * i);
+            sky_radiance(&a, T, S, cam, views + 3 * i, sun, l, rad + 3 * i, trans + 3 * i);
     }
 }
 
@@ -1235,7 +1259,7 @@ EXPORT void FN(oracle_render_frame)(const fb_parameters *p, const real *T, const
         real cam[3] = {camera[0], camera[1], camera[2]};
         real sn[3] = {sun[0], sun[1], sun[2]};
         real rad[3], t[3];
-        sky_radiance_to_point(&a, T, S, cam, view, world, sn, rad, t);
+        sky_radiance_to_point(&a, T, S, cam, view, world, sn, 0.0, rad, t);
         for (int c = 0; c < 3; ++c) {
             color[4 * idx + c] = rad[c];
             trans[4 * idx + c] = t[c];

@@ -108,6 +108,44 @@ def test_sky_rays_to_point_vs_oracle(builder, atm):
     assert e_rad.max() < 1e-3
 
 
+def test_light_shafts_vs_oracle(builder, atm):
+    """fb_render_sky_rays_shadowed (the light-shaft term the reference keeps
+    commented out, render_sky.h:81-106, :158-160, :172-177) against the FP32
+    restatement, with and without points; zero lengths reproduce
+    fb_render_sky_rays bit for bit."""
+    a, T, S, _ = atm
+    n = 4096
+    cams, views, suns = random_rays(n, 13)
+    cams[:256, 2] = 6500.0
+    rng = np.random.default_rng(14)
+    ell = rng.uniform(0.0, 20.0, n).astype(np.float32)
+    ell[::7] = 0.0
+    pts = (cams + views * rng.uniform(0.5, 50.0, n).astype(np.float32)[:, None]).astype(
+        np.float32)
+    r = Renderer(builder)
+    bufs = [DeviceBuffer.from_array(x) for x in (cams, views, suns, pts, ell)]
+    zero = DeviceBuffer.from_array(np.zeros(n, np.float32))
+    for points, floor in ((None, 1e-6), (bufs[3], 1e-4)):
+        outs = {}
+        for name, sh in (("plain", None), ("zero", zero), ("shadow", bufs[4])):
+            rad, tr = DeviceBuffer(n * 16), DeviceBuffer(n * 16)
+            r.rays(None, a, bufs[0], 1, bufs[1], bufs[2], 1, n, rad, tr, points=points,
+                   shadow_lengths=sh)
+            synchronize()
+            outs[name] = (rad.to_array(np.float32, (n, 4)), tr.to_array(np.float32, (n, 4)))
+        assert np.array_equal(outs["plain"][0], outs["zero"][0])
+        assert np.array_equal(outs["plain"][1], outs["zero"][1])
+        assert np.array_equal(outs["plain"][1], outs["shadow"][1])  # transmittance unchanged
+        o_rad, _ = Oracle("f32").sky_rays(P, T, S, cams, views, suns,
+                                          points=pts if points is not None else None,
+                                          shadow=ell)
+        e = rel_err(outs["shadow"][0], o_rad, floor)
+        print(f"\nlight shafts ({'to point' if points is not None else 'sky'}): "
+              f"max {e.max():.3e} p99.9 {np.quantile(e, 0.999):.3e}", flush=True)
+        assert np.quantile(e, 0.999) < 1e-4
+        assert e.max() < 1e-3
+
+
 def test_frames_match_rays(builder, atm):
     """fb_render_sky_frames == fb_render_sky_rays on the same hashed rays."""
     a, T, S, _ = atm

@@ -466,6 +466,48 @@ def test_sky_radiance_ray_missing_atmosphere():
     assert np.all(rad == 0.0) and np.all(tr == 1.0)
 
 
+def test_light_shafts_identities():
+    """The light-shaft term (render_sky.h:81-106, :158-160, :172-177, kept
+    commented out by the reference; Bruneton & Neyret Eqs. 17-18), checked
+    against the unshadowed lookups it must reduce to:
+      * l = 0 is the unshadowed path, bit for bit;
+      * GetSkyRadiance with l = the unshadowed radiance seen from x + l v,
+        times T(x, x + l v) (the transmittance GetSkyRadianceToPoint reports
+        for the point x + l v);
+      * GetSkyRadianceToPoint with l = the unshadowed lookup to the point at
+        d - l, with the transmittance to the true point; l >= d leaves no
+        scattering."""
+    p = SMOKE
+    r = O64.precompute(p.replace(order=3))
+    T, S = r["T"], r["S"]
+    rng = np.random.default_rng(12)
+    n = 256
+    v = rng.standard_normal((n, 3))
+    v[:, 2] = np.abs(v[:, 2]) + 0.05  # upward rays: none meets the ground
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    cams = np.stack([np.zeros(n), np.zeros(n), 6360.0 + rng.uniform(0.0, 5.0, n)], axis=1)
+    suns = np.stack([np.sin(0.7), 0.0, np.cos(0.7)])[None].repeat(n, 0)
+    ell = rng.uniform(0.5, 5.0, n)
+    base, _ = O64.sky_rays(p, T, S, cams, v, suns)
+    zero, _ = O64.sky_rays(p, T, S, cams, v, suns, shadow=np.zeros(n))
+    assert np.array_equal(base, zero)
+    got, tr_got = O64.sky_rays(p, T, S, cams, v, suns, shadow=ell)
+    moved, _ = O64.sky_rays(p, T, S, cams + v * ell[:, None], v, suns)
+    _, t_l = O64.sky_rays(p, T, S, cams, v, suns, points=cams + v * ell[:, None])
+    np.testing.assert_allclose(got, t_l * moved, rtol=1e-9, atol=1e-15)
+    assert np.array_equal(tr_got, O64.sky_rays(p, T, S, cams, v, suns)[1])
+    # to a point at distance d
+    d = rng.uniform(6.0, 30.0, n)
+    pts = cams + v * d[:, None]
+    got, tr_got = O64.sky_rays(p, T, S, cams, v, suns, points=pts, shadow=ell)
+    short, _ = O64.sky_rays(p, T, S, cams, v, suns, points=cams + v * (d - ell)[:, None])
+    _, tr_full = O64.sky_rays(p, T, S, cams, v, suns, points=pts)
+    np.testing.assert_allclose(got, short, rtol=1e-9, atol=1e-15)
+    assert np.array_equal(tr_got, tr_full)
+    gone, _ = O64.sky_rays(p, T, S, cams, v, suns, points=pts, shadow=d + 1.0)
+    assert np.abs(gone).max() <= 1e-12 * np.abs(base).max()
+
+
 def test_sky_radiance_ground_ray_transmittance_zero():
     """render_sky.h:76-78: GetSkyRadiance reports T = 0 for ground rays."""
     p = SMOKE
