@@ -372,9 +372,18 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
         TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
         const float mu_s = float(c.mu_s), nu = float(c.nu);
         Sum<COMP> rs[3], ms[3];
+        // The row table's entries of sample i+1 are loaded while sample i
+        // is evaluated (scalar loads, waited for at their first use).
+        float4 tv_n = s_tv[0], gq_n = s_geo[0], lq_n = s_lut[0], iv_n = s_inv[0];
+        float4 tr4_n = s_trw[0];
         for (int i = 0; i < kMsSamples; ++i) {
-            const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i], iv = s_inv[i];
-            const float4 tr4 = s_trw[i];
+            const float4 tv = tv_n, gq = gq_n, lq = lq_n, iv = iv_n, tr4 = tr4_n;
+            const int in = i + 1 < kMsSamples ? i + 1 : i;
+            tv_n = s_tv[in];
+            gq_n = s_geo[in];
+            lq_n = s_lut[in];
+            iv_n = s_inv[in];
+            tr4_n = s_trw[in];
             // The sample's two T rows (r taps) as one buffer: row 0 at byte 0,
             // row 1 at `drow` (0 when both taps are the same row).
             const int row0 = __float_as_int(tr4.x), drow = (__float_as_int(tr4.y) - row0) * 16;
@@ -1074,10 +1083,9 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
             return Rgb{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
         }
     };
-    // Issue sample i's loads; they stay in flight until put() reads them.
-    auto advance = [&](int i) {
-        const int mask = __float_as_int(d_geo(i).y);
-        const int4 o = d_off(i);
+    // Issue the loads of the sample whose slot mask and row offsets are
+    // (mask, o); they stay in flight until put() reads them.
+    auto advance = [&](int mask, int4 o) {
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (mask & 1) slot[q][0] = ld(o.x, q);
@@ -1152,16 +1160,23 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
         }
     };
     // (Each thread stages exactly its own texels x = tid + q*nt: W <= TPT*nt.)
-    advance(0);
+    advance(__float_as_int(d_geo(0).y), d_off(0));
     put_all(0, s_row);
     __syncthreads();
+    // The slot mask and row offsets of the sample after next are read (scalar
+    // loads) one iteration ahead of their use.
+    int mask_n = __float_as_int(d_geo(1).y);
+    int4 o_n = d_off(1);
     for (int i = 0; i < kMsSamples; ++i) {
         const Rgb* cur = s_row + (i & 1) * WP;
         Rgb* nxt = s_row + ((i + 1) & 1) * WP;
         const bool next = i + 1 < kMsSamples;
         // The loads of sample i+1 are issued first, so that their latency
         // overlaps the evaluation of sample i.
-        if (next) advance(i + 1);
+        if (next) advance(mask_n, o_n);
+        const int i2 = i + 2 < kMsSamples ? i + 2 : kMsSamples - 1;
+        mask_n = __float_as_int(d_geo(i2).y);
+        o_n = d_off(i2);
         eval(i, cur);
         if (next) put_all(i + 1, nxt);
         __syncthreads();
