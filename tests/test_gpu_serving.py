@@ -4,7 +4,7 @@ north-star check that each result is the same alone and in the mix."""
 import numpy as np
 import pytest
 
-from fuzzyblue_amd import Parameters, abi
+from fuzzyblue_amd import Atmosphere, Parameters, abi
 from fuzzyblue_amd.params import DUMP_SIZES, RESOLUTIONS, SMOKE_SIZES
 from fuzzyblue_amd.serving import Server
 from fuzzyblue_amd.workloads import random_atmosphere
@@ -109,4 +109,32 @@ def test_render_waits_for_inflight_precompute(builder):
     assert infos[rid].status == abi.FB_OK
     img = s.render_result(rid, 32, 16)
     assert np.all(np.isfinite(img)) and img[..., :3].max() > 0.0
+    s.close()
+
+
+def test_binary16_storage_requests(builder):
+    """Binary16-storage sets (FB_FLAG_F16_STORAGE) served beside FP32 ones:
+    each result equals the same set built directly, and the server holds
+    the smaller resident footprint for the binary16 set."""
+    p32 = Parameters().replace(**DUMP_SIZES, order=3, flags=abi.FB_FLAG_PRECISE)
+    p16 = p32.replace(flags=abi.FB_FLAG_PRECISE | abi.FB_FLAG_F16_STORAGE)
+    s = Server(builder, 8 * GB, keep_outputs=True)
+    _, l32 = s.submit_precompute(p32)
+    _, l16 = s.submit_precompute(p16)
+    rid = s.submit_render(l16, 2.0, 0.6, 64, 32, 7)
+    s.drain()
+    assert all(i.status == abi.FB_OK for i in s.poll())
+    for p, lut in ((p32, l32), (p16, l16)):
+        pa = Atmosphere.build(builder, None, p)
+        pa.wait()
+        a = pa.atmosphere()
+        got = fetch(s, lut, p)
+        for k, name in ((abi.FB_LUT_TRANSMITTANCE, "transmittance"),
+                        (abi.FB_LUT_IRRADIANCE, "irradiance"),
+                        (abi.FB_LUT_SCATTERING, "scattering")):
+            assert np.array_equal(got[k], a.download(name)), (p.flags, name)
+        pa.close()
+    img = s.render_result(rid, 64, 32)
+    assert np.all(np.isfinite(img)) and img[..., :3].max() > 0.0
+    assert s.stats().hbm_in_use == p32.footprint()[0] + p16.footprint()[0]
     s.close()
