@@ -430,8 +430,8 @@ def main():
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: the workload is one atmosphere's LUT set on one "
-            "MI355X; multi-GPU runs are not part of this build (atmospheres shard with no "
-            "exchange, see DESIGN.md)", file=sys.stderr)
+            "MI355X; runs on several GPUs are not supported (atmospheres would shard "
+            "across GPUs with no exchange, one whole LUT set per GPU)", file=sys.stderr)
         sys.exit(2)
 
     b = Builder(0)
