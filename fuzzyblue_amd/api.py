@@ -250,6 +250,11 @@ class PendingAtmosphere:
         """Make `stream` wait for the precompute (:2142-2201)."""
         check(load().fb_pending_stream_wait(self.handle, _stream_handle(stream)))
 
+    def record_ready_event(self, event):
+        """Make an fb_event (a hipEvent_t handle) complete once the
+        precompute has completed (acquire_ownership, :2147-2152)."""
+        check(load().fb_pending_record_ready_event(self.handle, event))
+
     def relaunch(self, stream=None):
         check(load().fb_pending_relaunch(self.handle, _stream_handle(stream)))
 

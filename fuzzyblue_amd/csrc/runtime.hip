@@ -650,6 +650,21 @@ fb_status fb_pending_stream_wait(const fb_pending_atmosphere* pa, fb_stream cons
     return FB_OK;
 }
 
+fb_status fb_pending_record_ready_event(const fb_pending_atmosphere* pa, fb_event event) {
+    if (!pa || !event) {
+        set_error("NULL argument");
+        return FB_EINVAL;
+    }
+    fb_builder* b = pa->builder;
+    FB_HIP_TRY(hipSetDevice(b->device));
+    // The builder's private stream (graph capture also runs on it, under
+    // the same lock) waits for the precompute, then records the event.
+    std::lock_guard<std::mutex> g(b->mu);
+    FB_HIP_TRY(hipStreamWaitEvent(b->capture_stream, pa->done, 0));
+    FB_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(event), b->capture_stream));
+    return FB_OK;
+}
+
 fb_status fb_pending_assert_ready(fb_pending_atmosphere* pa, fb_atmosphere** out) {
     if (!pa || !out || !pa->atm) {
         set_error("NULL argument or consumed pending atmosphere");
@@ -886,6 +901,10 @@ fb_status fb_event_destroy(fb_event e) {
 }
 fb_status fb_event_record(fb_event e, fb_stream s) {
     FB_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(e), static_cast<hipStream_t>(s)));
+    return FB_OK;
+}
+fb_status fb_event_synchronize(fb_event e) {
+    FB_HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(e)));
     return FB_OK;
 }
 fb_status fb_event_elapsed_ms(fb_event start, fb_event stop, float* ms) {

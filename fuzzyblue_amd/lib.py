@@ -73,6 +73,7 @@ def load():
         "fb_pending_query": (S, [V]),
         "fb_pending_wait": (S, [V]),
         "fb_pending_stream_wait": (S, [V, V]),
+        "fb_pending_record_ready_event": (S, [V, V]),
         "fb_pending_assert_ready": (S, [V, C.POINTER(V)]),
         "fb_pending_destroy": (None, [V]),
         "fb_pending_lut": (S, [V, S, C.POINTER(abi.fb_lut_view)]),
@@ -132,6 +133,7 @@ def load():
         "fb_event_destroy": (S, [V]),
         "fb_event_record": (S, [V, V]),
         "fb_event_elapsed_ms": (S, [V, V, C.POINTER(C.c_float)]),
+        "fb_event_synchronize": (S, [V]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)

@@ -258,6 +258,14 @@ fb_status fb_pending_wait(const fb_pending_atmosphere *pa);
 fb_status fb_pending_stream_wait(const fb_pending_atmosphere *pa,
                                  fb_stream consumer);
 
+/* The same hand-over as an event the caller owns (acquire_ownership,
+ * src/precompute.rs:2147-2152, for callers that synchronise on events):
+ * event completes once the precompute has completed.  The library records
+ * it on a private stream after a wait for the precompute, so the caller's
+ * streams are not touched.  Re-recording it later moves it as usual. */
+fb_status fb_pending_record_ready_event(const fb_pending_atmosphere *pa,
+                                        fb_event event);
+
 /* PendingAtmosphere::assert_ready (src/precompute.rs:2209): consumes the
  * pending object, returns the atmosphere and releases the scratch LUTs.
  * Returns FB_EBUSY (and leaves the pending object intact) if the work has
@@ -506,6 +514,7 @@ fb_status fb_event_create(fb_event *out);
 fb_status fb_event_destroy(fb_event e);
 fb_status fb_event_record(fb_event e, fb_stream s);
 fb_status fb_event_elapsed_ms(fb_event start, fb_event stop, float *ms);
+fb_status fb_event_synchronize(fb_event e);
 
 const char *fb_status_string(fb_status s);
 const char *fb_last_error(void);

@@ -11,10 +11,13 @@ Tolerances (relative error |x-y| / max(|y|, floor), floor = FLOOR * max|y|):
     transmittance 1e-5, irradiance and scattering 1e-3 (the bar the metric
     is quoted at).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from fuzzyblue_amd import Atmosphere, Parameters, abi
+from fuzzyblue_amd.lib import check, load
 from fuzzyblue_amd.params import DUMP_SIZES, SMOKE_SIZES
 from oracle import Oracle
 
@@ -136,3 +139,23 @@ def test_relaunch_is_deterministic(builder):
         for k in first:
             assert np.array_equal(first[k], again[k]), k
     pa.close()
+
+
+def test_ready_event_hand_over(builder):
+    """fb_pending_record_ready_event (acquire_ownership as an event): the
+    caller's event completes once the precompute has, and a consumer that
+    waited on it reads the finished LUTs."""
+    lib = load()
+    ev = C.c_void_p()
+    check(lib.fb_event_create(C.byref(ev)))
+    pa = Atmosphere.build(builder, None, SMOKE)
+    pa.record_ready_event(ev)
+    check(lib.fb_event_synchronize(ev))
+    assert pa.is_ready()
+    ref = Atmosphere.build(builder, None, SMOKE)
+    ref.wait()
+    assert np.array_equal(pa.atmosphere().download("scattering"),
+                          ref.atmosphere().download("scattering"))
+    check(lib.fb_event_destroy(ev))
+    pa.close()
+    ref.close()
