@@ -86,6 +86,10 @@ def test_views_and_downloads(builder):
     h = a.download("scattering", fp16=True)
     w = a.download("scattering")
     assert h.dtype == np.float16 and np.array_equal(h.astype(np.float32), w)
+    # the captured schedule replays to the same bits
+    pa.relaunch(None)
+    pa.wait()
+    assert np.array_equal(a.download("scattering", fp16=True), h)
     pa.close()
 
 
