@@ -260,3 +260,19 @@ def test_sun_and_sky_irradiance_vs_oracle(builder, atm):
     d_d, d_s = Oracle("f64").sun_and_sky_irradiance(P, T, E, pts, nrm, suns)
     assert np.quantile(rel_err(g_d, d_d, 1e-6), 0.99) < 1e-4
     assert rel_err(g_s, d_s, 1e-6).max() < 1e-4
+
+
+def test_empty_inputs(builder, atm):
+    """Zero rays, frames or points enqueue nothing and succeed; an empty batch
+    builds nothing."""
+    a, _, _, _ = atm
+    r = Renderer(builder)
+    one = DeviceBuffer.from_array(np.zeros(3, np.float32))
+    out = DeviceBuffer(16)
+    r.rays(None, a, one, 0, one, one, 0, 0, out)
+    r.rays(None, a, one, 0, one, one, 0, 0, out, shadow_lengths=one)
+    r.frames(None, a, one, one, 0, 100, 1, out)
+    r.frames(None, a, one, one, 1, 0, 1, out)
+    r.sun_and_sky_irradiance(None, a, one, one, one, 0, 0, out, out)
+    synchronize()
+    assert Atmosphere.build_batch(builder, None, []) == []
