@@ -281,23 +281,30 @@ __global__ __launch_bounds__(256) void k_render_rays(DevParams P, const float4* 
     if (trans_out) trans_out[i] = make_float4(tr.x, tr.y, tr.z, 1.0f);
 }
 
-// Counter-based direction generator: SplitMix64 over (seed, frame, pixel,
-// attempt), Marsaglia's (1972) rejection method for a uniform point on the
-// sphere.  Only +, *, sqrt (IEEE) are used, so the host copy in
-// fb_ray_direction() produces identical bits.
-__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
+// Counter-based direction generator: a 32-bit integer hash (the
+// xorshift-multiply "lowbias32" of C. Wellons' hash search, 2018) of (seed,
+// frame, pixel, attempt), and Marsaglia's (1972) rejection method for a
+// uniform point on the sphere.  32-bit integer multiplies only (a 64-bit
+// hash costs several 32-bit multiplies per product), and only +, *, sqrt
+// (IEEE) in floating point, so the host copy in fb_ray_direction()
+// produces identical bits.
+__host__ __device__ inline uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
 }
 __host__ __device__ inline void ray_direction(uint64_t seed, uint32_t frame, uint32_t pixel,
                                               float out[3]) {
-    uint64_t key = splitmix64(seed ^ (uint64_t(frame) << 32 | pixel));
+    // Per frame (block-uniform), then per pixel.
+    const uint32_t fk = hash32(frame ^ hash32(uint32_t(seed) ^ hash32(uint32_t(seed >> 32))));
+    const uint32_t key = hash32((pixel * 0x9e3779b1u) ^ fk);
     for (uint32_t attempt = 0; attempt < 64; ++attempt) {
-        uint64_t h = splitmix64(key + attempt);
-        float v1 = float(uint32_t(h >> 40)) * (2.0f / 16777216.0f) - 1.0f;
-        float v2 = float(uint32_t((h >> 16) & 0xffffffu)) * (2.0f / 16777216.0f) - 1.0f;
+        const uint32_t h1 = hash32(key + 2u * attempt), h2 = hash32(key + 2u * attempt + 1u);
+        float v1 = float(h1 >> 8) * (2.0f / 16777216.0f) - 1.0f;
+        float v2 = float(h2 >> 8) * (2.0f / 16777216.0f) - 1.0f;
         float s = v1 * v1 + v2 * v2;
         if (s < 1.0f && s > 0.0f) {
             float q = 2.0f * sqrtf(1.0f - s);
