@@ -11,10 +11,12 @@ command buffer, benches/precompute.rs:138-148).  Inputs (the parameters) are
 resident; no host transfer is inside the timed region.
 
 Prints ONE JSON line.  Extra fields: latency percentiles of individual
-build requests, FP32-mode throughput, sky Mpixel/s (64 frames of
-1920x1080 rays, camera altitude U[0,10] km, sun zenith U[0,108] deg), the
-roofline of the dominant kernel (scattering density) and a CPU baseline
-(the FP32 restatement of the reference passes on the host cores).
+build requests, FP32-mode and binary16-storage throughput, the reference's
+own benchmark configuration (scattering 8x32x8x2, median of 100
+submit-and-wait repetitions), sky Mpixel/s (64 frames of 1920x1080 rays,
+camera altitude U[0,10] km, sun zenith U[0,108] deg), the roofline of the
+dominant kernel (scattering density) and a CPU baseline (the FP32
+restatement of the reference passes on the host cores).
 """
 from __future__ import annotations
 
@@ -111,6 +113,33 @@ def measure_latency(b, stream, p, n):
         pa.close()
     lat = np.array(lat)
     return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def measure_reference_bench(b, stream, reps=100, warmup=10):
+    """The reference's own benchmark configuration (benches/precompute.rs:
+    123-147: default parameters with scattering 8x32x8x2, one recorded
+    command buffer submitted and waited for per iteration): the captured
+    graph relaunched and waited for per iteration, 10 warm-up + 100 timed
+    repetitions, median wall time (host clock around relaunch -> complete)."""
+    out = {"scattering_r_mu_mus_nu": [8, 32, 8, 2], "order": 4, "reps": reps}
+    base = Parameters().replace(scattering_r_size=8, scattering_mu_size=32,
+                                scattering_mu_s_size=8, scattering_nu_size=2)
+    for name, flags in (("fp32", 0), ("precise", abi.FB_FLAG_PRECISE)):
+        pa = Atmosphere.build(b, stream, base.replace(flags=flags))
+        pa.wait()
+        for _ in range(warmup):
+            pa.relaunch(stream)
+            stream.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            pa.relaunch(stream)
+            stream.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        pa.close()
+        med = float(np.median(ts))
+        out[name] = {"median_ms": med, "lut_sets_s": 1e3 / med}
+    return out
 
 
 def density_traffic():
@@ -488,6 +517,10 @@ def main():
                    "scattering_lut_bytes": pa16.atmosphere().lut("scattering").bytes}
     pa16.close()
 
+    log("[bench] the reference's benchmark configuration")
+    ref_bench = measure_reference_bench(b, stream)
+    log(f"[bench] reference-bench sizes: {ref_bench}")
+
     sky = None
     if not args.no_render:
         log("[bench] render")
@@ -551,6 +584,7 @@ def main():
         "max_rel_err_vs_fp64": err,
         "fp32_mode_lut_sets_s": args.steps / (ms32 * 1e-3),
         "f16_storage": f16_storage,
+        "reference_bench_config": ref_bench,
         "sky": sky,
         "batch_mixed256": batch,
         "serving": serving,
