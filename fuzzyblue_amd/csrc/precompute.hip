@@ -122,10 +122,26 @@ struct Sum {
 // transmittance.comp:8-46,71-80.  LANES lanes per texel: lane s sums the
 // trapezoid samples k = s, s+LANES, ... of all three optical lengths (the
 // reference recomputes r_i per profile with identical inputs), then the
-// partial sums are combined with lane shuffles.  The FP32 mode uses one
-// lane, i.e. the reference's sequential summation order, so its rounding
-// matches the reference's; the precision mode splits the sum over 16 lanes
-// with compensated partial sums.
+// partial sums are combined with lane shuffles: the precision mode, 16
+// lanes with compensated partial sums.  The FP32 mode keeps the reference's
+// sequential summation order, so its rounding matches the reference's
+// (transmittance_block_ordered below).
+// T = exp(-(beta_R L_R + beta_M,ext L_M + beta_abs L_O)) (transmittance.comp:
+// 34-46) for texel (i, j); with OD, also the optical depth itself, for
+// transmittances below binary32's range (see log_transmittance_to_top).
+__device__ __forceinline__ void store_transmittance(const DevParams& P, int i, int j, float LR,
+                                                    float LM, float LO, float4* __restrict__ T,
+                                                    float4* __restrict__ OD) {
+    const float t0 = P.rayleigh_scattering[0] * LR + P.mie_extinction[0] * LM +
+                     P.absorption_extinction[0] * LO;
+    const float t1 = P.rayleigh_scattering[1] * LR + P.mie_extinction[1] * LM +
+                     P.absorption_extinction[1] * LO;
+    const float t2 = P.rayleigh_scattering[2] * LR + P.mie_extinction[2] * LM +
+                     P.absorption_extinction[2] * LO;
+    T[j * P.t_mu + i] = make_float4(expf(-t0), expf(-t1), expf(-t2), 1.0f);
+    if (OD) OD[j * P.t_mu + i] = make_float4(t0, t1, t2, 0.0f);
+}
+
 template <typename G, bool COMP, int LANES>
 __device__ __forceinline__ void transmittance_block(const DevParams& P, const int bx, const int by, const int bz, float4* __restrict__ T, float4* __restrict__ OD) {
     const int sub = threadIdx.x % LANES;
@@ -160,16 +176,65 @@ __device__ __forceinline__ void transmittance_block(const DevParams& P, const in
         LO += __shfl_xor(LO, off, 64);
     }
     if (!active || sub != 0) return;
-    const float t0 = P.rayleigh_scattering[0] * LR + P.mie_extinction[0] * LM +
-                     P.absorption_extinction[0] * LO;
-    const float t1 = P.rayleigh_scattering[1] * LR + P.mie_extinction[1] * LM +
-                     P.absorption_extinction[1] * LO;
-    const float t2 = P.rayleigh_scattering[2] * LR + P.mie_extinction[2] * LM +
-                     P.absorption_extinction[2] * LO;
-    T[j * P.t_mu + i] = make_float4(expf(-t0), expf(-t1), expf(-t2), 1.0f);
-    // Precision mode: the optical depth itself, for transmittances below
-    // binary32's range (see log_transmittance_to_top).
-    if (OD) OD[j * P.t_mu + i] = make_float4(t0, t1, t2, 0.0f);
+    store_transmittance(P, i, j, LR, LM, LO, T, OD);
+}
+
+// Lanes per texel of the FP32-mode transmittance (a wavefront; the kernels
+// pick transmittance_block_ordered for this lane count).
+constexpr int kOrderedLanes = 64;
+
+// FP32 mode: the reference's summation order — per profile, s = s + term
+// for k = 0..500 in sequence (transmittance.comp:16-31) — with the terms
+// computed by a wavefront's 64 lanes in parallel (each term by the same
+// binary32 operations as in the sequential loop) and summed in order, one
+// lane per profile.  Bit-identical to one lane doing all of it.
+template <typename G>
+__device__ __forceinline__ void transmittance_block_ordered(const DevParams& P, const int bx,
+                                                            float4* __restrict__ T) {
+    constexpr int SAMPLE_COUNT = 500;
+    // Per wavefront of a 256-thread block; rows padded to 504 floats so that
+    // each starts 16-byte aligned (read as float4 below).
+    __shared__ __align__(16) float s_terms[4][3][504];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int texel = bx * int(blockDim.x / 64) + wave;
+    const int n = P.t_mu * P.t_r;
+    const bool active = texel < n;
+    const int i = active ? texel % P.t_mu : 0, j = active ? texel / P.t_mu : 0;
+    G x_mu = G(i) / G(P.t_mu - 1);
+    G x_r = G(j) / G(P.t_r - 1);
+    G r, mu;
+    rmu_from_transmittance_unit<G>(P, x_mu, x_r, r, mu);
+    G dx = distance_to_top<G>(P, r, mu) / G(SAMPLE_COUNT);
+    float dxf = float(dx);
+    for (int k = lane; k <= SAMPLE_COUNT; k += 64) {
+        G d_i = G(k) * dx;
+        G r_i = sqrt(d_i * d_i + G(2) * r * mu * d_i + r * r);
+        float alt = float(r_i - G(P.bottom));
+        float w = (k == 0 || k == SAMPLE_COUNT) ? 0.5f : 1.0f;
+        s_terms[wave][0][k] = profile_density(P.prof[0], alt) * w * dxf;
+        s_terms[wave][1][k] = profile_density(P.prof[1], alt) * w * dxf;
+        s_terms[wave][2][k] = profile_density(P.prof[2], alt) * w * dxf;
+    }
+    __syncthreads();
+    float L = 0.0f;
+    if (lane < 3) {
+        // Terms 0..499 four at a time (still added one by one, in order),
+        // then term 500.
+        const float* tp = s_terms[wave][lane];
+        const float4* tp4 = reinterpret_cast<const float4*>(tp);
+#pragma unroll 5
+        for (int q = 0; q < SAMPLE_COUNT / 4; ++q) {
+            const float4 v = tp4[q];
+            L = L + v.x;
+            L = L + v.y;
+            L = L + v.z;
+            L = L + v.w;
+        }
+        L = L + tp[SAMPLE_COUNT];
+    }
+    const float LR = __shfl(L, 0, 64), LM = __shfl(L, 1, 64), LO = __shfl(L, 2, 64);
+    if (!active || lane != 0) return;
+    store_transmittance(P, i, j, LR, LM, LO, T, nullptr);
 }
 
 // ====================================================== 2. direct irradiance
@@ -1222,7 +1287,10 @@ __device__ __forceinline__ LutPtrs lut_ptrs(const LutPtrs* L, int atm) {
 template <typename G, bool COMP, int LANES>
 __global__ __launch_bounds__(256) void k_transmittance(DevParams P, float4* __restrict__ T,
                                                        float4* __restrict__ OD) {
-    transmittance_block<G, COMP, LANES>(P, blockIdx.x, 0, 0, T, OD);
+    if constexpr (LANES == kOrderedLanes)
+        transmittance_block_ordered<G>(P, blockIdx.x, T);
+    else
+        transmittance_block<G, COMP, LANES>(P, blockIdx.x, 0, 0, T, OD);
 }
 template <typename G, bool COMP, int LANES>
 __global__ __launch_bounds__(256) void kb_transmittance(const DevParams* __restrict__ Ps,
@@ -1236,7 +1304,10 @@ __global__ __launch_bounds__(256) void kb_transmittance(const DevParams* __restr
         int atm, local;
         decode_tile(tiles[t], atm, local);
         const LutPtrs l = lut_ptrs(L, atm);
-        transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, l.T, l.OD);
+        if constexpr (LANES == kOrderedLanes)
+            transmittance_block_ordered<G>(Ps[atm], local, l.T);
+        else
+            transmittance_block<G, COMP, LANES>(Ps[atm], local, 0, 0, l.T, l.OD);
     }
 }
 
@@ -1452,7 +1523,8 @@ hipError_t launch_transmittance(const DevParams& P, float4* T, float4* OD, hipSt
     if (P.flags & 1)
         k_transmittance<double, true, 16><<<cdiv(texels * 16, 256), 256, 0, s>>>(P, T, OD);
     else
-        k_transmittance<float, false, 1><<<cdiv(texels, 64), 64, 0, s>>>(P, T, nullptr);
+        k_transmittance<float, false, kOrderedLanes>
+            <<<cdiv(texels * kOrderedLanes, 256), 256, 0, s>>>(P, T, nullptr);
     return hipGetLastError();
 }
 
@@ -1647,7 +1719,7 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
     const bool precise = P.flags & 1;
     switch (pass) {
         case kPassTransmittance: {
-            const size_t lanes = precise ? 16 : 1;
+            const size_t lanes = precise ? 16 : kOrderedLanes;
             return {cdiv(size_t(P.t_mu) * P.t_r * lanes, 256), 256, 0};
         }
         case kPassDirect:
@@ -1686,7 +1758,8 @@ static hipError_t launch_batch_h(int pass, int order, bool precise, const DevPar
             if (precise)
                 kb_transmittance<double, true, 16><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_transmittance<float, false, 1><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+                kb_transmittance<float, false, kOrderedLanes>
+                    <<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassDirect:
             if (precise)
@@ -1762,7 +1835,7 @@ static unsigned batch_resident_blocks_h(int pass, int order, bool precise, unsig
     switch (pass) {
         case kPassTransmittance:
             fn = precise ? (const void*)kb_transmittance<double, true, 16>
-                         : (const void*)kb_transmittance<float, false, 1>;
+                         : (const void*)kb_transmittance<float, false, kOrderedLanes>;
             break;
         case kPassDirect:
             fn = precise ? (const void*)kb_direct_irradiance<double>
