@@ -439,6 +439,8 @@ def main():
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-reference-config", action="store_true",
+                    help="skip the reference's own benchmark configuration (8x32x8x2)")
     ap.add_argument("--serving-rates", default="50",
                     help="comma-separated Poisson rates (req/s) for the serving run; "
                          "empty to skip")
@@ -517,9 +519,11 @@ def main():
                    "scattering_lut_bytes": pa16.atmosphere().lut("scattering").bytes}
     pa16.close()
 
-    log("[bench] the reference's benchmark configuration")
-    ref_bench = measure_reference_bench(b, stream)
-    log(f"[bench] reference-bench sizes: {ref_bench}")
+    ref_bench = None
+    if not args.no_reference_config:
+        log("[bench] the reference's benchmark configuration")
+        ref_bench = measure_reference_bench(b, stream)
+        log(f"[bench] reference-bench sizes: {ref_bench}")
 
     sky = None
     if not args.no_render:
