@@ -32,8 +32,12 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-def load():
+def load(path: str | None = None):
+    """The oracle library; ``path`` loads another build of the same source
+    (the mutation tests build copies with one planted mistake each)."""
     global _lib
+    if path is not None:
+        return C.CDLL(path)
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
@@ -48,12 +52,12 @@ def _ptr(a: np.ndarray):
 class Oracle:
     """One precision of the oracle.  Arrays are numpy float64/float32 RGBA."""
 
-    def __init__(self, precision: str = "f64"):
+    def __init__(self, precision: str = "f64", lib_path: str | None = None):
         assert precision in ("f64", "f32")
         self.precision = precision
         self.dtype = np.float64 if precision == "f64" else np.float32
         self.ctype = C.c_double if precision == "f64" else C.c_float
-        self.lib = load()
+        self.lib = load(lib_path)
         self._sig()
 
     def _fn(self, name):
@@ -85,6 +89,8 @@ class Oracle:
             "oracle_sample3d": (None, [V, C.c_int, C.c_int, C.c_int, R, R, R, V]),
             "oracle_get_transmittance": (None, [P, V, R, R, R, C.c_int, V]),
             "oracle_get_scattering": (None, [P, V, R, R, R, R, C.c_int, V]),
+            "oracle_density_at": (None, [P, C.c_int, V, V, V, V, V, R, R, R, R, V]),
+            "oracle_extrapolated_single_mie": (None, [P, V, V]),
             "oracle_pass_transmittance": (None, [P, V]),
             "oracle_pass_direct_irradiance": (None, [P, V, V]),
             "oracle_pass_single_scattering": (None, [P, C.c_int, V, V, V, V]),
@@ -231,6 +237,26 @@ class Oracle:
             C.byref(c), _ptr(t), r, mu, mu_s, nu, int(ground), _ptr(out)
         )
         return out
+
+    def density_at(self, p, order, T, dR, dM, dMS, dE, r, mu, mu_s, nu):
+        c = self.cparams(p)
+        args = [self.asarr(x) for x in (T, dR, dM, dMS, dE)]
+        out = self.arr(3)
+        self._fn("oracle_density_at")(
+            C.byref(c), int(order), *[_ptr(a) for a in args], r, mu, mu_s, nu, _ptr(out)
+        )
+        return out
+
+    def extrapolated_single_mie(self, p, s):
+        """render_sky.h:9-19 over an (..., 4) array of scattering values."""
+        c = self.cparams(p)
+        s = self.asarr(s)
+        flat = s.reshape(-1, 4)
+        out = self.arr(flat.shape[0], 3)
+        f = self._fn("oracle_extrapolated_single_mie")
+        for i in range(flat.shape[0]):
+            f(C.byref(c), _ptr(flat[i]), _ptr(out[i]))
+        return out.reshape(s.shape[:-1] + (3,))
 
     # ------------------------------------------------------------ passes
     def pass_transmittance(self, p):

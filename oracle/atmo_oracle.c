@@ -1081,6 +1081,23 @@ EXPORT void FN(oracle_get_scattering)(const fb_parameters *p, const real *tex, r
     load_atmo(&a, p, 0);
     get_scattering_tex(&a, tex, r, mu, mu_s, nu, ground, out);
 }
+/* The scattering density of scattering_density.comp:10-107 at an arbitrary
+ * (r, mu, mu_s, nu), not only at texel centres. */
+EXPORT void FN(oracle_density_at)(const fb_parameters *p, int order, const real *T,
+                                  const real *dR, const real *dM, const real *dMS,
+                                  const real *dE, real r, real mu, real mu_s, real nu,
+                                  real *out) {
+    atmo_t a;
+    load_atmo(&a, p, 0);
+    compute_scattering_density(&a, T, dR, dM, dMS, dE, r, mu, mu_s, nu, order, out);
+}
+/* render_sky.h:9-19 on one RGBA scattering value. */
+EXPORT void FN(oracle_extrapolated_single_mie)(const fb_parameters *p, const real *s,
+                                               real *out) {
+    atmo_t a;
+    load_atmo(&a, p, 0);
+    extrapolated_single_mie(&a, s, out);
+}
 
 /* Full passes over whole LUTs, in the reference's order
  * (src/precompute.rs:1726-1999). */

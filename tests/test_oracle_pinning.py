@@ -528,3 +528,32 @@ def test_sun_and_sky_irradiance_geometry():
     assert np.all(direct == 0.0) and np.all(sky == 0.0)
     direct, sky = O64.sun_and_sky_irradiance(p, r["T"], r["E"], [pt], [[0.0, 0.0, 1.0]], [sun])
     assert np.all(direct > 0.0) and np.all(sky > 0.0)
+
+
+# ------------------------------------------- closed forms (oracle_checks)
+import oracle_checks as _chk  # noqa: E402
+
+
+def test_transmittance_segments_constant_density():
+    """transmittance.h:35-61, ground and sky branches: exp(-beta d)."""
+    _chk.check_transmittance_segments(O64)
+
+
+def test_nu_lerp_reproduces_affine_lut():
+    """scattering.h:139-155: exact on a LUT affine in texel coordinates."""
+    _chk.check_nu_lerp_affine(O64)
+
+
+def test_order1_scattering_indirect_closed_form():
+    """scattering.h:157-179 order 1 through indirect_irradiance.comp:10-44."""
+    _chk.check_order1_indirect(O64)
+
+
+def test_order1_scattering_density_closed_form():
+    """scattering.h:157-179 order 1 through scattering_density.comp:10-107."""
+    _chk.check_order1_density(O64)
+
+
+def test_mie_extrapolation_identical_profiles():
+    """render_sky.h:9-19,21-43: identical profiles give delta_M back."""
+    _chk.check_mie_extrapolation(O64)
