@@ -194,7 +194,10 @@ __device__ void sky_radiance_slice(const DevParams& P, const float4* __restrict_
         trans = make_float3(t.x, t.y, t.z);
     }
     const Taps ty = linear_taps<float>(scattering_u_mu<float>(P, r, mu, ground), MU);
-    const float tex_coord_x = (nu + 1.0f) / 2.0f * float(NU - 1);
+    // nu of unit vectors can round just below -1 (anti-solar rays): clamped
+    // so the slice index stays >= 0.  The per-ray lookup's CLAMP_TO_EDGE
+    // then gives the same value up to a weight of the rounding's size.
+    const float tex_coord_x = (fminf(fmaxf(nu, -1.0f), 1.0f) + 1.0f) / 2.0f * float(NU - 1);
     const float tex_x = floorf(tex_coord_x);
     const float l = tex_coord_x - tex_x;
     const int k0 = int(tex_x);
@@ -446,8 +449,8 @@ static fb_status check_atm(fb_builder* b, const fb_atmosphere* a) {
     return FB_OK;
 }
 
-// Order the render after the precompute and remember it as the
-// atmosphere's last use (destroy waits for it).
+// Order the render after the precompute, and record it among the
+// atmosphere's uses (destroy waits for every recorded use, on any stream).
 static fb_status begin_use(fb_atmosphere* a, hipStream_t s) {
     FB_HIP_TRY(hipSetDevice(a->builder->device));
     FB_HIP_TRY(hipStreamWaitEvent(s, a->ready, 0));
@@ -455,9 +458,7 @@ static fb_status begin_use(fb_atmosphere* a, hipStream_t s) {
 }
 static fb_status end_use(fb_atmosphere* a, hipStream_t s) {
     FB_HIP_TRY(hipGetLastError());
-    FB_HIP_TRY(hipEventRecord(a->last_use, s));
-    a->used = true;
-    return FB_OK;
+    return atmosphere_mark_use(a, s);
 }
 
 extern "C" {

@@ -186,6 +186,38 @@ void free_atmosphere_buffers(fb_atmosphere* a) {
     free_buf(b, &a->S);
 }
 
+static bool event_done(hipEvent_t e) {
+    hipError_t q = hipEventQuery(e);
+    if (q == hipErrorNotReady) (void)hipGetLastError();
+    return q != hipErrorNotReady;  // an error also ends the wait for it
+}
+
+fb_status atmosphere_mark_use(fb_atmosphere* a, hipStream_t s) {
+    std::lock_guard<std::mutex> g(a->use_mu);
+    // Reuse the event of a render that has finished; otherwise add one, so
+    // renders in flight on different streams each keep their own event.
+    hipEvent_t ev = nullptr;
+    for (hipEvent_t e : a->uses)
+        if (event_done(e)) {
+            ev = e;
+            break;
+        }
+    if (!ev) {
+        FB_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        a->uses.push_back(ev);
+    }
+    FB_HIP_TRY(hipEventRecord(ev, s));
+    return FB_OK;
+}
+
+bool atmosphere_idle(fb_atmosphere* a) {
+    if (!event_done(a->ready)) return false;
+    std::lock_guard<std::mutex> g(a->use_mu);
+    for (hipEvent_t e : a->uses)
+        if (!event_done(e)) return false;
+    return true;
+}
+
 fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_a,
                      fb_pending_atmosphere** out_pa) {
     Sizes s = sizes_of(p);
@@ -215,15 +247,12 @@ fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_
     hipError_t e = hipSuccess;
     if (st == FB_OK) e = hipEventCreateWithFlags(&pa->done, hipEventDisableTiming);
     if (st == FB_OK && e == hipSuccess) e = hipEventCreateWithFlags(&a->ready, hipEventDisableTiming);
-    if (st == FB_OK && e == hipSuccess)
-        e = hipEventCreateWithFlags(&a->last_use, hipEventDisableTiming);
     if (st == FB_OK && e != hipSuccess) st = hip_fail(e, "hipEventCreate");
     if (st != FB_OK) {
         free_pending_scratch(pa);
         free_atmosphere_buffers(a);
         if (pa->done) (void)hipEventDestroy(pa->done);
         if (a->ready) (void)hipEventDestroy(a->ready);
-        if (a->last_use) (void)hipEventDestroy(a->last_use);
         delete pa;
         delete a;
         builder_release(b);
@@ -406,8 +435,17 @@ fb_status fb_parameters_validate(const fb_parameters* p) {
     }
     uint64_t ns = uint64_t(p->scattering_nu_size) * p->scattering_mu_s_size *
                   p->scattering_mu_size * p->scattering_r_size;
-    if (ns > (uint64_t(1) << 31)) {
-        set_error("scattering LUT of %llu texels is too large", (unsigned long long)ns);
+    // The kernels address one scattering-sized LUT, and the transmittance
+    // LUT, with 32-bit byte offsets (buffer resources, int texel offsets).
+    const uint64_t s_bytes = ns * ((p->flags & FB_FLAG_F16_STORAGE) ? 8u : 16u);
+    if (s_bytes >= (uint64_t(1) << 31)) {
+        set_error("scattering LUT of %llu texels (%llu bytes) exceeds 2 GiB",
+                  (unsigned long long)ns, (unsigned long long)s_bytes);
+        return FB_EINVAL;
+    }
+    const uint64_t t_bytes = uint64_t(p->transmittance_mu_size) * p->transmittance_r_size * 16u;
+    if (t_bytes >= (uint64_t(1) << 31)) {
+        set_error("transmittance LUT of %llu bytes exceeds 2 GiB", (unsigned long long)t_bytes);
         return FB_EINVAL;
     }
     if (p->order < 1 || p->order > 64) {
@@ -545,7 +583,6 @@ fb_status fb_atmosphere_build(fb_builder* b, fb_stream stream, const fb_paramete
         free_atmosphere_buffers(a);
         (void)hipEventDestroy(pa->done);
         (void)hipEventDestroy(a->ready);
-        (void)hipEventDestroy(a->last_use);
         delete pa;
         delete a;
         builder_release(b);
@@ -821,10 +858,16 @@ void fb_atmosphere_destroy(fb_atmosphere* a) {
     if (!a) return;
     (void)hipSetDevice(a->builder->device);
     (void)hipEventSynchronize(a->ready);
-    if (a->used) (void)hipEventSynchronize(a->last_use);
+    {
+        std::lock_guard<std::mutex> g(a->use_mu);
+        for (hipEvent_t e : a->uses) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
+        a->uses.clear();
+    }
     free_atmosphere_buffers(a);
     (void)hipEventDestroy(a->ready);
-    (void)hipEventDestroy(a->last_use);
     fb_builder* b = a->builder;
     delete a;
     builder_release(b);

@@ -100,9 +100,12 @@ struct fb_atmosphere {
     fb_parameters params{};
     fbk::DevParams dev{};
     fbrt::Buf T, E, S;
-    hipEvent_t ready = nullptr;     // precompute complete
-    hipEvent_t last_use = nullptr;  // last render enqueued against it
-    bool used = false;
+    hipEvent_t ready = nullptr;  // precompute complete
+    // Renders enqueued against it, possibly on several streams: one event
+    // per render still in flight (events of finished renders are reused).
+    // Destroy waits for every one of them before the LUTs go back to the pool.
+    std::mutex use_mu;
+    std::vector<hipEvent_t> uses;
 };
 
 struct fb_pending_atmosphere {
@@ -138,4 +141,9 @@ fb_status build_batch(fb_builder* b, hipStream_t s, const fb_parameters* params,
 fb_status batch_relaunch(fb_pending_atmosphere* pa, hipStream_t s);
 void batch_detach(fb_pending_atmosphere* pa);
 void free_atmosphere_buffers(fb_atmosphere* a);
+// Record a render of `a` enqueued on `s` (after its kernels).
+fb_status atmosphere_mark_use(fb_atmosphere* a, hipStream_t s);
+// True when the precompute and every recorded render of `a` have finished
+// (hipEventQuery only: never blocks).
+bool atmosphere_idle(fb_atmosphere* a);
 }  // namespace fbrt

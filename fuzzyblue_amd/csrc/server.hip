@@ -12,6 +12,11 @@
 //      and the HBM budget allow — evicting least-recently-used resident LUT
 //      sets that no render pins — and launches every admitted group of the
 //      same precision as ONE batch (one launch per pass over all of them).
+// A render pins its LUT set from submission until it finishes, so a set that
+// a queued render needs is never evicted under it.  An evicted set leaves
+// the resident map at once; its memory stays counted until its last use on
+// the GPU has finished (polled with hipEventQuery in step 1), so an eviction
+// never blocks the loop.
 // Renders have a device stream of their own and precomputes rotate over the
 // others.  With precompute_share_pct below 100 every precompute launch is
 // persistent and sized to that share of the device's resident-block slots,
@@ -66,6 +71,11 @@ struct Resident {
     uint64_t last_use = 0;
 };
 
+struct Evicted {
+    fb_atmosphere* atm = nullptr;
+    size_t bytes = 0;
+};
+
 }  // namespace
 
 struct fb_server {
@@ -82,7 +92,9 @@ struct fb_server {
     std::vector<fb_request_info> finished;
     std::unordered_map<uint64_t, Resident> resident;
     std::unordered_map<uint64_t, Req*> building;  // lut_id -> in-flight precompute
-    std::unordered_map<uint64_t, int> pins;       // lut_id -> renders in flight
+    std::unordered_map<uint64_t, int> pins;       // lut_id -> renders queued or in flight
+    std::vector<Evicted> evicted;                 // destroyed once idle on the GPU
+    size_t evicted_bytes = 0;
     std::unordered_map<uint64_t, std::pair<float4*, size_t>> outputs;
     size_t in_use = 0, peak = 0;
     uint64_t next_id = 1, next_lut = 1, clock = 0;
@@ -114,6 +126,7 @@ struct fb_server {
         info.t_start = r->t_start;
         info.t_done = now_s();
         finished.push_back(info);
+        if (r->kind == FB_REQUEST_RENDER) unpin(r->lut_id);
         if (st == FB_OK)
             stats.completed++;
         else
@@ -121,29 +134,56 @@ struct fb_server {
         delete r;
     }
 
-    // Evict LRU resident sets (not pinned, not `keep`) until `need` fits.
+    void unpin(uint64_t lut) {
+        auto it = pins.find(lut);
+        if (it != pins.end() && --it->second <= 0) pins.erase(it);
+    }
+    bool pinned(uint64_t lut) const {
+        auto it = pins.find(lut);
+        return it != pins.end() && it->second > 0;
+    }
+
+    // Destroy evicted sets whose GPU work has finished (never waits).
+    void reap_evicted() {
+        std::vector<Evicted> still;
+        for (const Evicted& e : evicted) {
+            if (!atmosphere_idle(e.atm)) {
+                still.push_back(e);
+                continue;
+            }
+            fb_atmosphere_destroy(e.atm);  // idle: returns without waiting
+            evicted_bytes -= e.bytes;
+            account(0, e.bytes);
+        }
+        evicted.swap(still);
+    }
+
+    // Evict LRU resident sets (not pinned, not `keep`) until `need` will fit
+    // once the evicted sets are idle; true when it fits now.
     bool make_room(size_t need, uint64_t keep) {
-        while (in_use + need > cfg.hbm_budget_bytes) {
+        reap_evicted();
+        while (in_use - evicted_bytes + need > cfg.hbm_budget_bytes) {
             uint64_t victim = 0, oldest = UINT64_MAX;
             for (auto& kv : resident) {
-                if (kv.first == keep || pins[kv.first] > 0) continue;
+                if (kv.first == keep || pinned(kv.first)) continue;
                 if (kv.second.last_use < oldest) {
                     oldest = kv.second.last_use;
                     victim = kv.first;
                 }
             }
-            if (!victim) return false;
+            if (!victim) break;
             Resident res = resident[victim];
             resident.erase(victim);
-            pins.erase(victim);
-            fb_atmosphere_destroy(res.atm);
-            account(0, res.bytes);
+            evicted.push_back(Evicted{res.atm, res.bytes});
+            evicted_bytes += res.bytes;
             stats.evictions++;
         }
-        return true;
+        reap_evicted();
+        return in_use + need <= cfg.hbm_budget_bytes;
     }
 
     void retire() {
+        reap_evicted();
         std::vector<Req*> still;
         for (Req* r : inflight) {
             if (r->kind == FB_REQUEST_PRECOMPUTE) {
@@ -187,7 +227,6 @@ struct fb_server {
                 (void)hipEventDestroy(r->ev);
                 (void)hipEventDestroy(r->ev0);
                 (void)hipEventDestroy(r->ev_adm);
-                pins[r->lut_id]--;
                 if (cfg.keep_outputs && e == hipSuccess) {
                     outputs[r->id] = {r->out, r->out_bytes};
                 } else {
@@ -252,7 +291,8 @@ struct fb_server {
                 e = fbk::launch_render_frame(atm->dev, atm->T.p, atm->S.p, r->alt, r->zen,
                                              r->w * r->h, r->seed, static_cast<float4*>(out), s);
             if (e == hipSuccess) e = hipEventRecord(r->ev, s);
-            if (e == hipSuccess) e = hipEventRecord(atm->last_use, s);
+            fb_status mst = e == hipSuccess ? atmosphere_mark_use(atm, s) : FB_OK;
+            if (e == hipSuccess && mst != FB_OK) e = hipErrorUnknown;
             if (e != hipSuccess) {
                 (void)hipStreamSynchronize(s);
                 if (r->ev) (void)hipEventDestroy(r->ev);
@@ -262,11 +302,9 @@ struct fb_server {
                 finish(r, hip_fail(e, "render launch"));
                 continue;
             }
-            atm->used = true;
             r->out = static_cast<float4*>(out);
             r->out_bytes = bytes;
             r->t_start = now_s();
-            pins[r->lut_id]++;
             account(bytes, 0);
             inflight.push_back(r);
         }
@@ -350,7 +388,7 @@ struct fb_server {
                 stats.step_max_ms[k] = std::max(stats.step_max_ms[k], float(1e3 * steps[k]));
                 if (steps[k] > 2e-3) stats.slow_steps++;
             }
-            if (inflight.empty() && !waiting.empty()) {
+            if (inflight.empty() && evicted.empty() && !waiting.empty()) {
                 // Nothing runs, so nothing will free memory: what is still
                 // waiting cannot fit the budget.
                 for (Req* r : waiting) {
@@ -359,7 +397,7 @@ struct fb_server {
                 }
                 waiting.clear();
             }
-            if (!inflight.empty()) {
+            if (!inflight.empty() || !evicted.empty()) {
                 // Poll completions; new submissions wake the loop early.
                 cv_work.wait_for(lk, std::chrono::microseconds(50));
             }
@@ -414,6 +452,7 @@ void fb_server_destroy(fb_server* s) {
     s->worker.join();  // the loop drains every submitted request first
     (void)hipSetDevice(s->b->device);
     for (auto& kv : s->resident) fb_atmosphere_destroy(kv.second.atm);
+    for (auto& e : s->evicted) fb_atmosphere_destroy(e.atm);
     for (auto& kv : s->outputs) s->b->pool->release(kv.second.first, kv.second.second);
     for (auto st : s->streams) (void)hipStreamDestroy(st);
     (void)hipStreamDestroy(s->clock_stream);
@@ -465,6 +504,7 @@ fb_status fb_server_submit_render(fb_server* s, uint64_t lut_id, float altitude_
     {
         std::lock_guard<std::mutex> g(s->mu);
         r->id = s->next_id++;
+        s->pins[lut_id]++;  // until the render finishes, whatever its outcome
         s->waiting.push_back(r);
         s->stats.submitted++;
         if (request_id) *request_id = r->id;

@@ -115,6 +115,29 @@ def test_validation_accepts_defaults_and_smoke_sizes():
         check(load().fb_parameters_validate(C.byref(p.to_c())))
 
 
+def test_validation_rejects_luts_beyond_32bit_offsets():
+    """The kernels address a scattering-sized LUT and the transmittance LUT
+    with 32-bit byte offsets: sets whose LUTs reach 2 GiB are rejected
+    (FB_EINVAL), in both storage modes; the largest set below the limit is
+    accepted."""
+    lib = load()
+    # 64 x 512 x 64 x 16 texels = 2^25: 512 MiB FP32, accepted
+    ok = Parameters().replace(scattering_r_size=64, scattering_mu_size=512,
+                              scattering_mu_s_size=64, scattering_nu_size=16)
+    check(lib.fb_parameters_validate(C.byref(ok.to_c())))
+    # 2^27 texels: exactly 2 GiB as RGBA32F, 1 GiB as binary16
+    big = ok.replace(scattering_mu_size=2048)
+    assert lib.fb_parameters_validate(C.byref(big.to_c())) == abi.FB_EINVAL
+    assert b"2 GiB" in lib.fb_last_error()
+    check(lib.fb_parameters_validate(C.byref(big.replace(flags=abi.FB_FLAG_F16_STORAGE).to_c())))
+    bigger = big.replace(scattering_mu_size=4096, flags=abi.FB_FLAG_F16_STORAGE)
+    assert lib.fb_parameters_validate(C.byref(bigger.to_c())) == abi.FB_EINVAL
+    # transmittance: 65536 x 2048 texels x 16 B = 2 GiB
+    t = Parameters().replace(transmittance_mu_size=65536, transmittance_r_size=2048)
+    assert lib.fb_parameters_validate(C.byref(t.to_c())) == abi.FB_EINVAL
+    check(lib.fb_parameters_validate(C.byref(t.replace(transmittance_r_size=2047).to_c())))
+
+
 def test_negative_coefficient_rejected():
     p = Parameters()
     p.rayleigh_scattering = [-1.0, 0.0, 0.0]

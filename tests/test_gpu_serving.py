@@ -138,3 +138,29 @@ def test_binary16_storage_requests(builder):
     assert np.all(np.isfinite(img)) and img[..., :3].max() > 0.0
     assert s.stats().hbm_in_use == p32.footprint()[0] + p16.footprint()[0]
     s.close()
+
+
+def test_queued_renders_pin_their_set(builder):
+    """Renders pin their LUT set from submission: under a budget tight enough
+    that renders queue for output memory while later precomputes need room,
+    no render finds its set evicted (FB_ENOENT), and evicted sets are still
+    freed so every request completes."""
+    p = Parameters().replace(**RESOLUTIONS["half"])
+    res, scratch = p.footprint()
+    w, h = 640, 360
+    out = w * h * 16
+    s = Server(builder, 3 * res + scratch + 3 * out, max_inflight=1)
+    luts = [s.submit_precompute(p.replace(order=2 + (k % 3)))[1] for k in range(5)]
+    rids = []
+    for k, lut in enumerate(luts):
+        for j in range(3):
+            rids.append(s.submit_render(lut, 1.0 + j, 0.2 * j, w, h, 100 * k + j))
+    s.drain()
+    infos = {i.id: i for i in s.poll()}
+    assert len(infos) == len(luts) + len(rids)
+    bad = {i.id: i.status for i in infos.values() if i.status != abi.FB_OK}
+    assert not bad, bad
+    st = s.stats()
+    assert st.evictions >= 1
+    assert st.hbm_peak <= 3 * res + scratch + 3 * out
+    s.close()
