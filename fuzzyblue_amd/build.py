@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libfuzzyblue_hip.so")
 ARCH = os.environ.get("FB_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["precompute.hip", "render.hip", "runtime.hip", "batch.hip", "server.hip",
-           "tune.hip"]
+           "tune.hip", "noalloc.hip"]
 
 COMMON = [
     f"--offload-arch={ARCH}",

@@ -546,6 +546,7 @@ fb_status fb_builder_create(int hip_device, fb_builder** out) {
         return st;
     }
     b->pool = std::make_unique<Pool>(hip_device);
+    noalloc_init(hip_device);
     *out = b;
     return FB_OK;
 }

@@ -146,4 +146,7 @@ fb_status atmosphere_mark_use(fb_atmosphere* a, hipStream_t s);
 // True when the precompute and every recorded render of `a` have finished
 // (hipEventQuery only: never blocks).
 bool atmosphere_idle(fb_atmosphere* a);
+// Look up this device's copy of the no-allocation build's static scratch
+// (noalloc.hip); called once per device by fb_builder_create.
+void noalloc_init(int device);
 }  // namespace fbrt

@@ -63,6 +63,9 @@ def load():
         "fb_builder_destroy": (None, [V]),
         "fb_atmosphere_build": (S, [V, V, P, C.POINTER(V)]),
         "fb_atmosphere_build_batch": (S, [V, V, P, C.c_size_t, V]),
+        "fb_atmosphere_build_into": (
+            S, [V, V, P, V, C.c_size_t, V, C.c_size_t, V, C.c_size_t]),
+        "fb_build_into_scratch_limit": (C.c_size_t, []),
         "fb_pending_relaunch": (S, [V, V]),
         "fb_builder_autotune": (S, [V, V, P, C.POINTER(abi.fb_tuning)]),
         "fb_builder_get_tuning": (S, [V, P, C.POINTER(abi.fb_tuning)]),

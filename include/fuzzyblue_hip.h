@@ -170,6 +170,26 @@ fb_status fb_atmosphere_build_batch(fb_builder *b, fb_stream stream,
                                     const fb_parameters *params, size_t n,
                                     fb_pending_atmosphere **out);
 
+/* The precompute of one atmosphere into caller-owned output LUTs, with no
+ * device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, serving
+ * loops and harnesses that forbid hipMalloc).  The caller passes the three
+ * output LUTs (transmittance and irradiance RGBA32F, scattering RGBA32F or,
+ * with FB_FLAG_F16_STORAGE, RGBA binary16, in the layouts above) with their
+ * sizes in bytes; every transient LUT and table lives in a static device
+ * array of fb_build_into_scratch_limit() bytes (96 MiB: the default LUT sizes
+ * fit, larger sets return FB_EINVAL), one per GPU, held from the first
+ * fb_builder_create on that GPU until the process ends.  Enqueue only; no
+ * event or object is created, so the caller orders later work with its own
+ * stream.  Calls that overlap on one GPU share the array and race: enqueue
+ * them on one stream.  LUTs are bit-identical to fb_atmosphere_build's. */
+fb_status fb_atmosphere_build_into(fb_builder *b, fb_stream stream,
+                                   const fb_parameters *params,
+                                   void *transmittance, size_t transmittance_bytes,
+                                   void *irradiance, size_t irradiance_bytes,
+                                   void *scattering, size_t scattering_bytes);
+size_t fb_build_into_scratch_limit(void);
+
 /* Re-run the recorded precompute of a pending atmosphere into the same
  * buffers (the reference's bench re-submits one recorded command buffer:
  * benches/precompute.rs:138-148).  Uses the captured hipGraph. */
