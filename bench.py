@@ -2,21 +2,25 @@
 """Benchmark: precompute LUT sets/s (+ p50/p99 latency) at <= 1e-3 relative
 error vs the FP64 oracle, on one MI355X.
 
-Workload (one "step"): the whole precompute of one default Earth atmosphere
-(Parameters::default(), src/precompute.rs:849-935: transmittance 256x64,
-irradiance 64x16, scattering 32x128x32x8, order 4) through the plain
-single-atmosphere HIP pipeline in precision mode, re-launched into the same
-resident buffers (the reference's bench likewise re-submits one recorded
-command buffer, benches/precompute.rs:138-148).  Inputs (the parameters) are
-resident; no host transfer is inside the timed region.
+Workload (one "step"): the seeded 256-atmosphere mixed set
+(workloads.mixed_batch(256, seed=4): bottom radius 2000-10000 km, thickness
+20-200 km, Rayleigh / Mie 1e-4 - 1e-1 /km, LUT resolution half / default /
+double, orders 2-8), precision mode, submitted at once to the serving loop
+(fb_server_*: continuous batching, 200 GB HBM budget) and drained.  `value`
+is LUT sets per second over the timed steps and `latency_ms` the p50 / p99
+of the requests' submission-to-completion times.  Parameters are 252-byte
+host structs; the LUTs stay resident in HBM, no host transfer is timed.
 
-Prints ONE JSON line.  Extra fields: latency percentiles of individual
-build requests, FP32-mode and binary16-storage throughput, the reference's
-own benchmark configuration (scattering 8x32x8x2, median of 100
-submit-and-wait repetitions), sky Mpixel/s (64 frames of 1920x1080 rays,
-camera altitude U[0,10] km, sun zenith U[0,108] deg), the roofline of the
-dominant kernel (scattering density) and a CPU baseline (the FP32
-restatement of the reference passes on the host cores).
+Prints ONE JSON line.  Extra fields: the same set as one batch and through
+the plain pipeline (each atmosphere's own schedule, back to back), 256
+half-resolution atmospheres likewise, the worst FP64-oracle error per
+resolution class of the mixed-set members that have fixtures (built as one
+batch), the plain default Earth set (relaunch throughput, latency, FP32
+mode, binary16 storage), the reference's own benchmark configuration
+(scattering 8x32x8x2), sky Mpixel/s (64 frames of 1920x1080 rays), the
+measured VALU and LDS ceilings and the scattering-density roofline, the
+precision workload's errors, the Poisson serving sweep, and a CPU baseline
+(the FP32 restatement of the reference passes on the host cores).
 """
 from __future__ import annotations
 
@@ -56,7 +60,7 @@ METRIC = "LUT-sets/s + p50/p99 latency at ≤1e-3 rel err vs FP64, 1×MI355X"
 PROFILE_STEPS = 10
 # rocprofv3 --pmc summary of the density kernel on this workload (FETCH_SIZE
 # and WRITE_SIZE per dispatch, calibrated against a plain copy).
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "scattering_density_default_earth_hbm.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "scattering_density_double_earth_hbm.txt")
 
 
 def log(msg):
@@ -144,8 +148,8 @@ def measure_reference_bench(b, stream, reps=100, warmup=10):
 
 def density_traffic():
     """HBM bytes per density launch (average over the order-2 and the two
-    order>=3 launches of one LUT set) from the committed rocprofv3 --pmc
-    summary; None if it is absent."""
+    order>=3 launches of one "double" Earth LUT set) from the committed
+    rocprofv3 --pmc summary; None if it is absent."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             for line in f:
@@ -197,51 +201,86 @@ def measure_render(b, stream, atm, frames=64, w=1920, h=1080, reps=3):
     return frames * px / (ms * 1e-3) / 1e6, ms
 
 
-def measure_batch(b, stream, n=256, seed=4):
-    """The seeded mixed set (workloads.mixed_batch: sizes half/default/double,
-    orders 2..8) as one batch (one launch per pass, graph replay) against the
-    plain pipeline (each atmosphere's own schedule), both timed with HIP
-    events on the same stream."""
-    from fuzzyblue_amd.workloads import mixed_batch
+def plain_back_to_back(b, stream, ps):
+    """The plain pipeline over a list of atmospheres: each one's own
+    schedule (its captured graph), all relaunched back to back on one stream
+    after every set is built once.  Returns ms for the whole list."""
+    pend = []
+    for p in ps:
+        pa = Atmosphere.build(b, stream, p)
+        pa.wait()
+        pend.append(pa)
+    for pa in pend:  # warm: every graph instantiated once
+        pa.relaunch(stream)
+    stream.synchronize()
+    t = Timer()
+    t.start(stream)
+    for pa in pend:
+        pa.relaunch(stream)
+    t.stop(stream)
+    ms = t.ms()
+    for pa in pend:
+        pa.close()
+    return ms
 
-    ps = mixed_batch(n, seed, precise=True)
-    # One autotune per LUT-size class of the mix (block sizes only; the
-    # LUTs are bit-identical for every choice).
-    for res in RESOLUTIONS.values():
-        b.autotune(Parameters().replace(**res, flags=abi.FB_FLAG_PRECISE), stream)
-    work = sum(costmodel.lut_set_work(p).roof_seconds() for p in ps)
+
+def batched(b, stream, ps, reps=2):
+    """One batch (one launch per pass over all members, graph replay):
+    ms per replay."""
     pend = Atmosphere.build_batch(b, stream, ps)
     for pa in pend:
         pa.wait()
     t = Timer()
-    reps = 2
     t.start(stream)
     for _ in range(reps):
         pend[0].relaunch(stream)
     t.stop(stream)
-    ms_batch = t.ms() / reps
+    ms = t.ms() / reps
     for pa in pend:
         pa.close()
-    ms_plain = 0.0
-    for p in ps:
-        pa = Atmosphere.build(b, stream, p)
-        pa.wait()
-        t.start(stream)
-        pa.relaunch(stream)
-        t.stop(stream)
-        ms_plain += t.ms()
-        pa.close()
-    return {
+    return ms
+
+
+def measure_batch(b, stream, ps, label):
+    """A set of atmospheres as one batch against the plain pipeline (each
+    atmosphere's own schedule), both timed with HIP events on one stream."""
+    n = len(ps)
+    work = sum(costmodel.lut_set_work(p).roof_seconds() for p in ps)
+    ms_batch = batched(b, stream, ps)
+    ms_plain = plain_back_to_back(b, stream, ps)
+    out = {
+        "workload": label,
         "atmospheres": n,
-        "seed": seed,
         "batched_lut_sets_s": n / (ms_batch * 1e-3),
         "plain_lut_sets_s": n / (ms_plain * 1e-3),
         "speedup_vs_plain": ms_plain / ms_batch,
         "batched_ms": ms_batch,
         "plain_ms": ms_plain,
-        "roofline_ms": work * 1e3,
-        "roofline_frac": work * 1e3 / ms_batch,
+        "survey_roofline_ms": work * 1e3,
     }
+    log(f"[bench] {label}: {out}")
+    return out
+
+
+def serve_all_at_once(b, ps, budget_gb=200.0, max_inflight=64):
+    """Submit every parameter set to a fresh serving loop at once and wait
+    for all of them: (wall seconds from first submission to the last
+    completion, per-request latencies in ms, failures)."""
+    from fuzzyblue_amd.serving import Server
+
+    srv = Server(b, int(budget_gb * 1e9), max_inflight=max_inflight, n_streams=4)
+    synchronize()
+    t0 = time.perf_counter()
+    for p in ps:
+        srv.submit_precompute(p)
+    srv.drain()
+    synchronize()
+    wall = time.perf_counter() - t0
+    infos = srv.poll()
+    lat = [1e3 * (i.t_done - i.t_submit) for i in infos if i.status == abi.FB_OK]
+    failed = sum(1 for i in infos if i.status != abi.FB_OK)
+    srv.close()
+    return wall, lat, failed
 
 
 def _rel(x, y, scale):
@@ -301,54 +340,147 @@ def measure_precision(b, stream):
     return out
 
 
-def measure_mixed_accuracy(b, stream):
-    """Eight random atmospheres drawn like the mixed set (half resolution,
-    orders 2-8) as one precision-mode batch against their FP64-oracle LUTs:
-    the worst member's max relative error per LUT."""
+def mixed_fixture_members():
+    """Members of the mixed workload with FP64-oracle fixtures: eight "half"
+    atmospheres drawn like the set (mixed_half_fp64.npz) and six members of
+    the seeded 256-atmosphere set itself at the "default" and "double"
+    resolutions (mixed_large_fp64.npz).  Returns [(class, params, fixture
+    arrays T, E, S, S_idx)]."""
     import json as _json
 
-    path = os.path.join(ROOT, "tests", "golden", "mixed_half_fp64.npz")
-    if not os.path.exists(path):
+    out = []
+    for name in ("mixed_half_fp64.npz", "mixed_large_fp64.npz"):
+        path = os.path.join(ROOT, "tests", "golden", name)
+        if not os.path.exists(path):
+            continue
+        g = np.load(path)
+        ps = [Parameters.from_dict(d) for d in _json.loads(str(g["params"]))]
+        for i, p in enumerate(ps):
+            idx = g[f"S_idx{i}"] if f"S_idx{i}" in g else g["S_idx"]
+            cls = {4: "half", 8: "default", 16: "double"}[p.scattering_nu_size]
+            out.append((cls, p, g[f"T{i}"], g[f"E{i}"], g[f"S{i}"], idx))
+    return out
+
+
+def measure_mixed_accuracy(b, stream):
+    """Every member with an FP64-oracle fixture (half, default and double
+    resolution, orders 2-8) built as ONE precision-mode batch: the worst
+    member's max relative error per LUT, per resolution class
+    (|x - x64| / max(|x64|, 1e-6 max|LUT64|) per channel)."""
+    members = mixed_fixture_members()
+    if not members:
         return None
-    g = np.load(path)
-    ps = [Parameters.from_dict(d) for d in _json.loads(str(g["params"]))]
-    pend = Atmosphere.build_batch(b, stream, ps)
-    worst = {"transmittance": 0.0, "irradiance": 0.0, "scattering": 0.0}
-    for i, pa in enumerate(pend):
+    pend = Atmosphere.build_batch(b, stream, [m[1] for m in members])
+    worst = {}
+    for (cls, p, T64, E64, S64, idx), pa in zip(members, pend):
         pa.wait()
         a = pa.atmosphere()
-        for name, key, x in (("transmittance", "T", a.download("transmittance")[..., :3]),
-                             ("irradiance", "E", a.download("irradiance")[..., :3]),
-                             ("scattering", "S",
-                              a.download("scattering").reshape(-1, 4)[g["S_idx"]])):
-            y = g[f"{key}{i}"]
+        w = worst.setdefault(cls, {"members": 0, "transmittance": 0.0, "irradiance": 0.0,
+                                   "scattering": 0.0})
+        w["members"] += 1
+        for name, x, y in (("transmittance", a.download("transmittance")[..., :3], T64),
+                           ("irradiance", a.download("irradiance")[..., :3], E64),
+                           ("scattering", a.download("scattering").reshape(-1, 4)[idx], S64)):
             scale = np.abs(y).reshape(-1, y.shape[-1]).max(axis=0)
-            worst[name] = max(worst[name], float(_rel(x, y, scale).max()))
+            w[name] = max(w[name], float(_rel(x, y, scale).max()))
     for pa in pend:
         pa.close()
-    return {"atmospheres": len(ps), "max_rel_err": worst}
+    return {"batch_of": len(members), "max_rel_err_by_class": worst}
 
 
-def measure_served_mixed(b, n=256, seed=4, budget_gb=200.0):
-    """The whole seeded mixed set submitted to the serving loop at once:
-    LUT sets per second from first submission to the last completion."""
-    from fuzzyblue_amd.serving import Server
-    from fuzzyblue_amd.workloads import mixed_batch
+CEILINGS_LIB = os.path.join(ROOT, "fuzzyblue_amd", "libfb_ceilings.so")
 
-    ps = mixed_batch(n, seed, precise=True)
-    srv = Server(b, int(budget_gb * 1e9), max_inflight=64, n_streams=4)
-    t0 = time.perf_counter()
-    for p in ps:
-        srv.submit_precompute(p)
-    srv.drain()
-    wall = time.perf_counter() - t0
-    infos = srv.poll()
-    lat = np.array([1e3 * (i.t_done - i.t_submit) for i in infos if i.status == abi.FB_OK])
-    failed = sum(1 for i in infos if i.status != abi.FB_OK)
-    srv.close()
-    return {"atmospheres": n, "lut_sets_s": float(lat.size / wall), "wall_s": wall,
-            "failed": int(failed), "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99))}
+
+def measure_ceilings():
+    """Measured gfx950 ceilings (fuzzyblue_amd/csrc/ceilings.hip): the
+    non-packed v_fma_f32 rate and the LDS read rate of b128 + b64 entry
+    reads (the density lookup) with 1, 8, 16 and 64 distinct entries per
+    wavefront read."""
+    import ctypes as C
+
+    lib = C.CDLL(CEILINGS_LIB)
+    lib.fb_ceiling_run.restype = C.c_int
+    lib.fb_ceiling_run.argtypes = [C.c_int] * 5 + [C.c_void_p, C.POINTER(C.c_float)]
+    lib.fb_ceiling_work.restype = C.c_double
+    lib.fb_ceiling_work.argtypes = [C.c_int] * 3
+    blocks = 256 * 32  # 32 blocks of 256 threads per CU: 8 waves per SIMD
+    out = DeviceBuffer(blocks * 256 * 4)
+
+    def run(kind, iters, distinct=8):
+        ms = C.c_float()
+        rc = lib.fb_ceiling_run(kind, blocks, iters, distinct, 5, out.ptr, C.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"ceiling probe {kind} failed: hipError {rc}")
+        return lib.fb_ceiling_work(kind, blocks, iters) / (ms.value * 1e-3)
+
+    fma = run(0, 64)
+    res = {"valu_fma_tflops": 2.0 * fma / 1e12,
+           "valu_fma_wave_instr_per_s": fma / 64.0,
+           "valu_fmac_wave_instr_per_s": run(4, 64) / 64.0,
+           "valu_add_wave_instr_per_s": run(5, 64) / 64.0,
+           "valu_mul_wave_instr_per_s": run(6, 64) / 64.0,
+           "valu_fma_vop3_wave_instr_per_s": run(8, 64) / 64.0,
+           "valu_med3_wave_instr_per_s": run(9, 64) / 64.0,
+           "valu_lshl_add_wave_instr_per_s": run(10, 64) / 64.0,
+           "valu_fma_sgpr_wave_instr_per_s": run(11, 64) / 64.0,
+           "valu_fma_inline_const_wave_instr_per_s": run(12, 64) / 64.0,
+           "valu_mul_sgpr_wave_instr_per_s": run(13, 64) / 64.0,
+           "valu_max_mul_wave_instr_per_s": run(14, 64) / 64.0,
+           "valu_add_fract_wave_instr_per_s": run(15, 64) / 64.0,
+           "probe": "fuzzyblue_amd/csrc/ceilings.hip, 8192 blocks x 256 threads, 5 launches"}
+    for d in (1, 8, 16, 64):
+        res[f"lds_b128_b64_tbps_distinct{d}"] = run(1, 64, d) / 1e12
+        res[f"lds_split_b128_b64_tbps_distinct{d}"] = run(7, 64, d) / 1e12
+    res["lds_b128_tbps_distinct8"] = run(2, 64, 8) / 1e12
+    res["lds_b64_tbps_distinct8"] = run(3, 64, 8) / 1e12
+    out.close()
+    return res
+
+
+def measure_density_roofline(b, stream, ceilings, steps=PROFILE_STEPS):
+    """The dominant kernel of the mixed workload, scattering density, on a
+    "double"-resolution Earth LUT set (scattering 64x256x64x16; the double
+    members are 94% of the mix's work), order 4, precision mode: per-launch
+    duration from HIP events on the launch stream (fb_pending_profile,
+    `steps` re-runs), against the LDS read ceiling (the kernel reads a
+    24-byte profile entry per sample: its binding resource)."""
+    p = Parameters().replace(**RESOLUTIONS["double"], flags=abi.FB_FLAG_PRECISE)
+    pa = Atmosphere.build(b, stream, p)
+    pa.wait()
+    pa.relaunch(stream)
+    stream.synchronize()
+    acc = {}
+    for _ in range(steps):
+        for name, order, t in pa.profile(stream):
+            if name == "scattering_density":
+                acc.setdefault(order, []).append(t)
+    pa.close()
+    launches = sorted(acc)
+    ms = [float(np.mean(acc[o])) for o in launches]
+    lds = [costmodel.density_lds_bytes(p, o) for o in launches]
+    hbm = [costmodel.pass_work(p, "scattering_density", o).bytes for o in launches]
+    avg_ms = float(np.mean(ms))
+    achieved = float(np.mean(lds)) / (avg_ms * 1e-3) / 1e12
+    peak = max(v for k, v in ceilings.items() if "b128_b64_tbps" in k)
+    return {
+        "bound": "lds",
+        "kernel": "k_scattering_density<double,*,false>",
+        "workload": "Earth, scattering 64x256x64x16, order 4, precision mode",
+        "achieved": achieved,
+        "peak": peak,
+        "unit": "TB/s",
+        "frac": achieved / peak,
+        "traffic": density_traffic(),
+        "traffic_source": "rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch, "
+                          "profiles/scattering_density_double_earth_hbm.txt",
+        "algorithmic_lds_bytes_per_launch": float(np.mean(lds)),
+        "algorithmic_hbm_bytes_per_launch": float(np.mean(hbm)),
+        "avg_launch_ms": avg_ms,
+        "launch_ms_by_order": dict(zip([str(o) for o in launches], ms)),
+        "profile_steps": steps,
+        "peak_source": "measured: max over access patterns of fb_ceiling_run b128+b64",
+        "hbm_frac": float(np.mean(hbm)) / (avg_ms * 1e-3) / (costmodel.PEAK_HBM_GBS * 1e9),
+    }
 
 
 def measure_serving(b, rates, duration, budget_gb=200.0):
@@ -370,13 +502,16 @@ def measure_serving(b, rates, duration, budget_gb=200.0):
     return {"duration_s": duration, "hbm_budget_gb": budget_gb, "runs": out}
 
 
-def cpu_baseline(p):
+def cpu_baseline(mix):
     """FP32 restatement of the reference passes on the host cores, timed on a
-    bounded sample: full transmittance / direct / indirect passes, and texel
-    subsets of single scattering, density and multiple scattering scaled to
-    the full LUT."""
+    bounded sample: the default Earth LUT set's full transmittance / direct /
+    indirect passes and texel subsets of single scattering, density and
+    multiple scattering, scaled to the full set; each member of the mixed
+    set is then charged that time times its cost-model FLOP ratio to the
+    default Earth set."""
     from oracle import Oracle
 
+    p = Parameters()
     o = Oracle("f32")
     nt, ni, ns = costmodel.texel_counts(p)
     rng = np.random.default_rng(0)
@@ -412,28 +547,79 @@ def cpu_baseline(p):
     t_M = (time.perf_counter() - t0) * ns / n_m
     n_orders = p.order - 1
     t_set = t_T + t_D + t_S + t_J2 + (n_orders - 1) * t_J3 + n_orders * (t_I + t_M)
+    ref_flop = costmodel.lut_set_work(p).flop
+    t_mix = sum(t_set * costmodel.lut_set_work(q).flop / ref_flop for q in mix)
     cores = len(os.sched_getaffinity(0))
     threads = int(os.environ.get("OMP_NUM_THREADS", cores))
     return {
-        "value": 1.0 / t_set,
+        "value": len(mix) / t_mix,
         "unit": "LUT-sets/s",
         "cores": min(cores, threads),
         "kind": "port",
         "sample": (
-            "FP32 restatement (oracle/atmo_oracle.c, OpenMP) of the reference passes for the "
+            "FP32 restatement (oracle/atmo_oracle.c, OpenMP) of the reference passes on the "
             "default Earth LUT set, order 4: full transmittance/direct/indirect passes; "
             f"{n_ss} single-scattering, {n_j} density (orders 2 and 3) and {n_m} "
-            "multiple-scattering texels, scaled to the full 1,048,576-texel LUT"
+            "multiple-scattering texels, scaled to the full 1,048,576-texel LUT "
+            f"({t_set:.1f} s per set), then to each of the {len(mix)} mixed-set members by "
+            "its cost-model FLOP ratio"
         ),
-        "seconds_per_set": t_set,
+        "seconds_per_default_set": t_set,
+        "seconds_per_mixed_set": t_mix,
     }
+
+
+def kernels_per_set(p) -> int:
+    """Kernel launches of one plain schedule: transmittance, geometry,
+    direct irradiance, single scattering, irradiance clear, then density,
+    indirect and multiple scattering per order 2..order."""
+    return 5 + 3 * (p.order - 1)
+
+
+def trace_summary(csv_path, last=0):
+    """Summary of a rocprofv3 kernel trace (--kernel-trace --output-format
+    csv), over its last `last` dispatches (all when 0): the device-idle
+    fraction 1 - (union of kernel intervals) / (first start -> last end),
+    and time, count and mean duration per kernel name."""
+    import csv
+
+    rows = []
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]),
+                         row["Kernel_Name"].split("(")[0]))
+    rows.sort()
+    if last:
+        rows = rows[-last:]
+    busy, cur_a, cur_e = 0, None, None
+    by_name = {}
+    for a, e, name in rows:
+        t, n = by_name.get(name, (0, 0))
+        by_name[name] = (t + e - a, n + 1)
+        if cur_e is None or a > cur_e:
+            if cur_e is not None:
+                busy += cur_e - cur_a
+            cur_a, cur_e = a, e
+        else:
+            cur_e = max(cur_e, e)
+    if cur_e is not None:
+        busy += cur_e - cur_a
+    span = rows[-1][1] - rows[0][0] if rows else 0
+    kern = sorted(by_name.items(), key=lambda kv: -kv[1][0])
+    return {"kernels": len(rows), "span_ms": span / 1e6, "busy_ms": busy / 1e6,
+            "idle_frac": 1.0 - busy / span if span else None,
+            "by_kernel": [{"name": k, "total_ms": t / 1e6, "count": n,
+                           "avg_us": t / n / 1e3} for k, (t, n) in kern]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2,
+                    help="timed steps; one step serves the whole 256-atmosphere mixed set")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=50,
+                    help="relaunches of the default Earth set (side measurement)")
     ap.add_argument("--latency-requests", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true")
@@ -441,14 +627,33 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-reference-config", action="store_true",
                     help="skip the reference's own benchmark configuration (8x32x8x2)")
-    ap.add_argument("--serving-rates", default="50",
-                    help="comma-separated Poisson rates (req/s) for the serving run; "
+    ap.add_argument("--serving-rates", default="10,50,200",
+                    help="comma-separated Poisson rates (req/s) for the serving sweep; "
                          "empty to skip")
-    ap.add_argument("--serving-seconds", type=float, default=10.0)
+    ap.add_argument("--serving-seconds", type=float, default=60.0)
+    ap.add_argument("--only", choices=["roofline", "plain-mix", "batch-mix", "served-mix",
+                                       "ceilings"],
+                    help="run one measurement alone (for profiler runs) and print its JSON")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="with --only: keep the default block shapes (shorter counter runs)")
+    ap.add_argument("--trace", metavar="CSV",
+                    help="summarise a rocprofv3 kernel-trace CSV (device-idle fraction, "
+                         "time per kernel) and exit (no GPU use)")
+    ap.add_argument("--trace-last", type=int, default=0,
+                    help="with --trace: only the last N dispatches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.trace:
+        print(json.dumps(trace_summary(args.trace, args.trace_last)))
+        return
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: the workload is a set of atmospheres on one "
+            "MI355X; runs on several GPUs are not supported (atmospheres would shard "
+            "across GPUs with no exchange, whole LUT sets per GPU)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         src = DeviceBuffer(args.copy_calibration)
         dst = DeviceBuffer(args.copy_calibration)
@@ -458,72 +663,118 @@ def main():
         synchronize()
         print(json.dumps({"copy_bytes": args.copy_calibration, "copies": 5}))
         return
-    if args.gpus != 1:
-        print(
-            "bench.py measures one GPU: the workload is one atmosphere's LUT set on one "
-            "MI355X; runs on several GPUs are not supported (atmospheres would shard "
-            "across GPUs with no exchange, one whole LUT set per GPU)", file=sys.stderr)
-        sys.exit(2)
+
+    from fuzzyblue_amd.workloads import mixed_batch
 
     b = Builder(0)
     stream = Stream()
-    p = Parameters().replace(flags=abi.FB_FLAG_PRECISE)
-    log("[bench] autotune")
-    tuning = b.autotune(p, stream)
-    tuning32 = b.autotune(p.replace(flags=0), stream)
-    log(f"[bench] tuning: {tuning}")
-    log("[bench] precision-mode throughput")
-    pa, ms = measure_throughput(b, stream, p, args.steps, args.warmup)
-    value = args.steps / (ms * 1e-3)
-    ms_per_step = ms / args.steps
-    log(f"[bench] {value:.1f} LUT-sets/s ({ms_per_step:.3f} ms/set)")
+    precise = abi.FB_FLAG_PRECISE
+    mix = mixed_batch(256, 4, precise=True)
+    # One autotune per LUT-size class (block shapes only: every choice gives
+    # bit-identical LUTs); builds of the class use it from then on.
+    tuning = {}
+    if not (args.only and args.no_autotune):
+        log("[bench] autotune")
+        for name, res in RESOLUTIONS.items():
+            tuning[name] = b.autotune(Parameters().replace(**res, flags=precise), stream)
 
-    # Per-launch profile of the same schedule (HIP events on the launch
-    # stream around each launch), over PROFILE_STEPS consecutive steps run
-    # right after the timed region.
-    profs = [pa.profile(stream) for _ in range(PROFILE_STEPS)]
-    prof = [(name, order, t / PROFILE_STEPS) for name, order, t in profs[0]]
-    for other in profs[1:]:
-        prof = [(name, order, t + u / PROFILE_STEPS)
-                for (name, order, t), (_, _, u) in zip(prof, other)]
-    passes = {}
-    for name, order, t in prof:
-        passes.setdefault(name, []).append(t)
-    dens = [(order, t) for name, order, t in prof if name == "scattering_density"]
-    dens_flop = sum(costmodel.pass_work(p, "scattering_density", o).flop for o, _ in dens)
-    dens_ms = sum(t for _, t in dens)
-    achieved = dens_flop / (dens_ms * 1e-3) / 1e12
-    dens_bytes = sum(costmodel.pass_work(p, "scattering_density", o).bytes for o, _ in dens)
+    if args.only:
+        if args.only == "ceilings":
+            out = measure_ceilings()
+        elif args.only == "roofline":
+            out = measure_density_roofline(b, stream, {"lds_b128_b64_tbps": 1.0})
+            # the profile re-runs are the last dispatches of this process
+            out["trace_last"] = PROFILE_STEPS * kernels_per_set(
+                Parameters().replace(**RESOLUTIONS["double"]))
+        elif args.only == "plain-mix":
+            out = {"plain_ms": plain_back_to_back(b, stream, mix),
+                   "trace_last": sum(kernels_per_set(q) for q in mix)}
+        elif args.only == "batch-mix":
+            out = {"batched_ms": batched(b, stream, mix, reps=1)}
+        else:
+            wall, lat, failed = serve_all_at_once(b, mix)
+            out = {"wall_s": wall, "failed": failed}
+        print(json.dumps(out), flush=True)
+        stream.close()
+        b.close()
+        return
 
-    err = max_err_vs_fixture(pa.atmosphere())
-    log(f"[bench] max rel err vs FP64 oracle: {err}")
+    # ---------------------------------------------------------- headline
+    # One step: the 256-atmosphere mixed set (sizes half/default/double,
+    # orders 2..8) submitted at once to the serving loop (continuous
+    # batching, 200 GB HBM budget) and drained; timed from the first
+    # submission to the last completion with the device idle on both sides.
+    log(f"[bench] served mixed set: {args.warmup} warm-up + {args.steps} timed steps")
+    for _ in range(args.warmup):
+        serve_all_at_once(b, mix)
+    walls, lats, failed = [], [], 0
+    for k in range(args.steps):
+        wall, lat, f = serve_all_at_once(b, mix)
+        walls.append(wall)
+        lats.extend(lat)
+        failed += f
+        log(f"[bench] step {k}: {len(mix) / wall:.1f} LUT-sets/s ({wall:.3f} s)")
+    total = float(np.sum(walls))
+    value = len(mix) * args.steps / total
+    lat_ms = {"p50": float(np.percentile(lats, 50)), "p99": float(np.percentile(lats, 99)),
+              "requests": len(lats), "failed": failed,
+              "what": "per precompute request, submission to completion, all 256 submitted "
+                      "at once per step"}
 
-    log("[bench] latency")
-    p50, p99 = measure_latency(b, stream, p, args.latency_requests)
+    batch = half = None
+    if not args.no_batch:
+        log("[bench] mixed set batched vs plain")
+        batch = measure_batch(b, stream, mix, "256-atmosphere mixed set (seed 4)")
+        batch["served_lut_sets_s"] = value
+        batch["served_speedup_vs_plain"] = value / batch["plain_lut_sets_s"]
+        from fuzzyblue_amd.workloads import random_atmosphere
 
-    log("[bench] fp32-mode throughput")
-    pa32, ms32 = measure_throughput(b, stream, p.replace(flags=0), args.steps, args.warmup)
+        rng = np.random.default_rng(4)
+        halves = [random_atmosphere(rng, True, resolution="half") for _ in range(256)]
+        half = measure_batch(b, stream, halves,
+                             "256 half-resolution atmospheres drawn like the mixed set")
+        wall, _, _ = serve_all_at_once(b, halves)
+        half["served_lut_sets_s"] = 256 / wall
+
+    log("[bench] mixed-set members vs FP64 fixtures")
+    mixed_acc = measure_mixed_accuracy(b, stream)
+    log(f"[bench] {mixed_acc}")
+
+    # ------------------------------------------------ side measurements
+    p = Parameters().replace(flags=precise)
+    log("[bench] plain default Earth set (relaunch)")
+    pa, ms = measure_throughput(b, stream, p, args.plain_steps, 5)
+    plain_default = {"lut_sets_s": args.plain_steps / (ms * 1e-3),
+                     "ms_per_set": ms / args.plain_steps,
+                     "max_rel_err_vs_fp64": max_err_vs_fixture(pa.atmosphere())}
+    prof = {}
+    for _ in range(PROFILE_STEPS):
+        for name, order, t in pa.profile(stream):
+            prof[name] = prof.get(name, 0.0) + t / PROFILE_STEPS
+    plain_default["passes_ms"] = prof
+    plain_default["latency_ms"] = dict(zip(("p50", "p99"),
+                                           measure_latency(b, stream, p, args.latency_requests)))
+    b.autotune(p.replace(flags=0), stream)
+    pa32, ms32 = measure_throughput(b, stream, p.replace(flags=0), args.plain_steps, 5)
+    plain_default["fp32_mode_lut_sets_s"] = args.plain_steps / (ms32 * 1e-3)
     pa32.close()
-
-    # The reference's storage formats (binary16 scattering and 3D scratch),
-    # precision-mode arithmetic, default block shapes.
-    log("[bench] binary16-storage throughput")
-    p16 = p.replace(flags=abi.FB_FLAG_PRECISE | abi.FB_FLAG_F16_STORAGE)
-    pa16, ms16 = measure_throughput(b, stream, p16, args.steps, args.warmup)
-    f16_storage = {"lut_sets_s": args.steps / (ms16 * 1e-3),
+    p16 = p.replace(flags=precise | abi.FB_FLAG_F16_STORAGE)
+    pa16, ms16 = measure_throughput(b, stream, p16, args.plain_steps, 5)
+    f16_storage = {"lut_sets_s": args.plain_steps / (ms16 * 1e-3),
                    "max_rel_err_vs_fp64": max_err_vs_fixture(pa16.atmosphere()),
                    # values below the binary16 range flush to 0 (floor 1e-6
                    # x max counts them at 1.0); with a floor of 1e-3 x max:
                    "max_rel_err_vs_fp64_floor_1e-3": max_err_vs_fixture(pa16.atmosphere(),
                                                                          1e-3),
-                   "scattering_lut_bytes": pa16.atmosphere().lut("scattering").bytes}
+                   "scattering_lut_bytes": pa16.atmosphere().lut("scattering").bytes,
+                   "note": "binary16 storage is the reference's precision, not the metric's"}
     pa16.close()
+    log(f"[bench] plain default: {plain_default}")
 
     ref_bench = None
     if not args.no_reference_config:
         log("[bench] the reference's benchmark configuration")
         ref_bench = measure_reference_bench(b, stream)
-        log(f"[bench] reference-bench sizes: {ref_bench}")
 
     sky = None
     if not args.no_render:
@@ -531,21 +782,17 @@ def main():
         mpx, rms = measure_render(b, stream, pa.atmosphere())
         sky = {"mpixel_s": mpx, "ms_per_64_frames": rms, "frames": 64, "width": 1920,
                "height": 1080}
+    pa.close()
 
-    batch = None
-    if not args.no_batch:
-        log("[bench] mixed batch of 256 atmospheres")
-        batch = measure_batch(b, stream)
-        batch["served"] = measure_served_mixed(b)
-        batch["served_speedup_vs_plain"] = (batch["served"]["lut_sets_s"] /
-                                            batch["plain_lut_sets_s"])
-        log(f"[bench] batch: {batch}")
+    log("[bench] ceilings and density roofline")
+    ceilings = measure_ceilings()
+    roofline = measure_density_roofline(b, stream, ceilings)
+    log(f"[bench] {ceilings} {roofline}")
 
-    precision = mixed = None
+    precision = None
     if not args.no_precision:
-        log("[bench] precision workload and mixed-set accuracy")
+        log("[bench] precision workload")
         precision = measure_precision(b, stream)
-        mixed = measure_mixed_accuracy(b, stream)
 
     serving = None
     if args.serving_rates:
@@ -556,9 +803,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
-        cpu = cpu_baseline(Parameters())
+        cpu = cpu_baseline(mix)
 
-    work = costmodel.lut_set_work(p)
     line = {
         "metric": METRIC,
         "value": value,
@@ -566,54 +812,38 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * total / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "plain single-atmosphere precompute of the default Earth atmosphere "
-                        "(order 4; LUTs 256x64, 64x16, 32x128x32x8), precision mode (FP64 "
-                        "geometry, FP32 radiance)",
-            "order": p.order,
-            "transmittance": list(p.transmittance_extent()),
-            "irradiance": list(p.irradiance_extent()),
-            "scattering_r_mu_mus_nu": [p.scattering_r_size, p.scattering_mu_size,
-                                       p.scattering_mu_s_size, p.scattering_nu_size],
-            "precision": "precise",
+            "workload": "256-atmosphere mixed set (seed 4): bottom radius 2000-10000 km, "
+                        "thickness 20-200 km, Rayleigh/Mie 1e-4-1e-1 /km, LUT resolution "
+                        "half/default/double, orders 2-8, precision mode; served through the "
+                        "continuous-batching loop (fb_server_*), 200 GB HBM budget, all 256 "
+                        "requests submitted at once per step",
+            "atmospheres": len(mix),
+            "resolutions": {k: sum(1 for q in mix if q.scattering_nu_size == v)
+                            for k, v in (("half", 4), ("default", 8), ("double", 16))},
+            "precision": "precise (FP64 geometry, FP32 radiance and storage)",
         },
-        "latency_ms": {"p50": p50, "p99": p99, "requests": args.latency_requests},
-        "tuning": {"precise": tuning, "fp32": tuning32},
-        "max_rel_err_vs_fp64": err,
-        "fp32_mode_lut_sets_s": args.steps / (ms32 * 1e-3),
+        "latency_ms": lat_ms,
+        "max_rel_err_vs_fp64": mixed_acc,
+        "batch_mixed256": batch,
+        "batch_half256": half,
+        "plain_default_earth": plain_default,
         "f16_storage": f16_storage,
         "reference_bench_config": ref_bench,
         "sky": sky,
-        "batch_mixed256": batch,
         "serving": serving,
         "precision_double_order8": precision,
-        "mixed_half_vs_fp64": mixed,
-        "roofline": {
-            "bound": "valu",
-            "kernel": "scattering_density",
-            "achieved": achieved,
-            "peak": costmodel.PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / costmodel.PEAK_FP32_TFLOPS,
-            "traffic": density_traffic(),
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch, "
-                              "profiles/scattering_density_default_earth_hbm.txt",
-            "algorithmic_bytes_per_launch": dens_bytes / max(len(dens), 1),
-            "avg_launch_ms": dens_ms / max(len(dens), 1),
-            "launches": len(dens),
-            "profile_steps": PROFILE_STEPS,
-        },
-        "lut_set_roofline_ms": work.roof_seconds() * 1e3,
-        "passes_ms": {k: float(np.sum(v)) for k, v in passes.items()},
+        "tuning": tuning,
+        "ceilings": ceilings,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    pa.close()
     stream.close()
     b.close()
     print(json.dumps(line), flush=True)

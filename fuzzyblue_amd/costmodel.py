@@ -109,3 +109,41 @@ def lut_set_work(p) -> Work:
         for name in ("scattering_density", "indirect_irradiance", "multiple_scattering"):
             w = w + pass_work(p, name, order)
     return w
+
+
+def density_ground_rows(p) -> list:
+    """Per r index z of the scattering LUT: how many of the 16 theta rows of
+    the density quadrature (theta_l = (l + 1/2) pi/16) meet the ground
+    (RayIntersectsGround, params.h:126-133) at the texel radius
+    r_z = sqrt(rho_z^2 + bottom^2), rho_z = H z/(R-1) (scattering.h:62-75)."""
+    import math
+
+    b, t = float(p.bottom_radius), float(p.top_radius)
+    H = math.sqrt(t * t - b * b)
+    R = p.scattering_r_size
+    out = []
+    for z in range(R):
+        rho = H * z / (R - 1)
+        r = math.sqrt(rho * rho + b * b)
+        n = 0
+        for l in range(16):
+            mu = math.cos((l + 0.5) * math.pi / 16)
+            if mu < 0.0 and r * r * (mu * mu - 1.0) + b * b >= 0.0:
+                n += 1
+        out.append(n)
+    return out
+
+
+def density_lds_bytes(p, order: int) -> float:
+    """LDS bytes the scattering-density kernel reads in its sample loop per
+    launch: each of the 16 x 32 direction samples of a texel reads one
+    (value, delta) profile entry of the incident radiance — 24 bytes (one
+    b128 + one b64 read) of multiple scattering, or 48 bytes (three b128) of
+    the single Rayleigh + Mie pair at order 2 — and, on theta rows that meet
+    the ground, one 24-byte entry of the ground irradiance."""
+    per = 48 if order == 2 else 24
+    texels_per_r = p.scattering_nu_size * p.scattering_mu_s_size * p.scattering_mu_size
+    total = 0.0
+    for g in density_ground_rows(p):
+        total += texels_per_r * 32 * (16 * per + g * 24)
+    return total
