@@ -124,7 +124,18 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
                          pass == fbk::kPassIndirect || pass == fbk::kPassMultiple;
     for (int i : order_idx) {
         if (!member(i)) continue;
-        if (pass == fbk::kPassMultiple) {
+        if (pass == fbk::kPassSingle) {
+            // The staged kernel (transmittance rows in LDS) or, for very wide
+            // transmittance LUTs, the direct one: one launch per kernel.
+            const unsigned c = fbk::ss_staged(dps[i]) ? 1u : 0u;
+            auto g = std::find_if(groups.begin(), groups.end(),
+                                  [&](const Group& x) { return x.tpt == c; });
+            if (g == groups.end())
+                groups.push_back(Group{fbk::tuned_threads(dps[i], pass, order, tus[i]), c,
+                                       bool((tus[i].remap >> pass) & 1u), {i}});
+            else
+                g->members.push_back(i);
+        } else if (pass == fbk::kPassMultiple) {
             const unsigned t = fbk::tuned_threads(dps[i], pass, order, tus[i]);
             const unsigned c = ms_tpt(dps[i], t);
             auto g = std::find_if(groups.begin(), groups.end(),
@@ -140,10 +151,11 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
             groups[0].members.push_back(i);
         }
     }
-    if (tunable && pass != fbk::kPassMultiple && !groups.empty())
-        for (int i : groups[0].members)
-            if (!fbk::tuning_valid(dps[i], pass, order, groups[0].threads))
-                groups[0].threads = pass == fbk::kPassIndirect ? 64 : 256;
+    if (tunable && pass != fbk::kPassMultiple)
+        for (Group& g : groups)
+            for (int i : g.members)
+                if (!fbk::tuning_valid(dps[i], pass, order, g.threads))
+                    g.threads = pass == fbk::kPassIndirect ? 64 : 256;
     for (Group& g : groups) {
         BatchLaunch l;
         l.pass = pass;

@@ -87,6 +87,16 @@ struct Tuning {
 // `order` is the scattering order of the launch: density n, the indirect
 // pass of the order-n iteration (n - 1 bounces), multiple n.
 unsigned default_threads(const DevParams& P, int pass, int order);
+// Single scattering stages the r-interpolated transmittance row of each
+// sample in LDS (two rows, double buffered) when the LUT is at most
+// kSsMaxStagedMu wide (64 KiB), with at most kSsTexelsPerThread texels per
+// thread; wider transmittance LUTs take the kernel that reads its taps
+// directly.
+constexpr int kSsTexelsPerThread = 4;
+constexpr int kSsMaxStagedMu = 2048;
+inline bool ss_staged(const DevParams& P) { return P.t_mu <= kSsMaxStagedMu; }
+inline size_t ss_staged_lds_bytes(int t_mu) { return 2 * size_t(t_mu) * 16; }
+
 unsigned tuned_threads(const DevParams& P, int pass, int order, const Tuning& tu);
 bool tuning_valid(const DevParams& P, int pass, int order, unsigned threads);
 

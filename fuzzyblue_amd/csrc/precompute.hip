@@ -407,7 +407,52 @@ __device__ __forceinline__ float3 lerp3(float3 a, float3 b, float w) {
     return make_float3(fmaf(w, b.x - a.x, a.x), fmaf(w, b.y - a.y, a.y), fmaf(w, b.z - a.z, a.z));
 }
 
-template <typename G, bool COMP, bool H>
+// The sun-direction coordinate of sample (tv, gq, lq, iv) of texel
+// (mu_s, nu): mu_s_d, the texel coordinate t of T(r_d, mu_s_d) along the
+// LUT's mu axis (clamped to [0, TMU-1]) and the smoothstep sun-disc factor
+// (transmittance.h:26-33, 63-74).
+template <bool PRECISE>
+__device__ __forceinline__ void ss_sun_coord(float rf, float mu_s, float nu, float4 tv, float4 gq,
+                                             float4 lq, float4 iv, float top2, int TMU,
+                                             float& t, float& sm) {
+    const float r_d = gq.x, d_i = tv.w;
+    const float tmax = float(TMU - 1);
+    float mu_s_d;
+    if constexpr (PRECISE) {
+        // Sun-ray distance to the top boundary without cancellation: for
+        // mu_s_d > 0 the root is (top^2 - r_d^2) / (r_d mu_s_d + sqrt(disc)),
+        // with top^2 - r_d^2 from the FP64 set-up.  Reciprocals of the
+        // per-sample divisors come from FP64 too.
+        mu_s_d = __builtin_amdgcn_fmed3f(fmaf(rf, mu_s, d_i * nu) * iv.x, -1.0f, 1.0f);
+        const float rm = r_d * mu_s_d;
+        const float sq = __builtin_amdgcn_sqrtf(fmaf(rm, rm, iv.z));
+        const float d = mu_s_d > 0.0f ? iv.z * __builtin_amdgcn_rcpf(rm + sq) : sq - rm;
+        // tex_from_unit then linear_taps: texel coordinate x_mu*(N-1).
+        t = __builtin_amdgcn_fmed3f((d - lq.y) * iv.y * tmax, 0.0f, tmax);
+    } else {
+        mu_s_d = clamp_cosine<float>((rf * mu_s + d_i * nu) / r_d);
+        float disc = r_d * r_d * (mu_s_d * mu_s_d - 1.0f) + top2;
+        float d = clamp_distance<float>(-r_d * mu_s_d + safe_sqrt<float>(disc));
+        float x_mu = (d - lq.y) / lq.z;
+        // linear_taps of tex_from_unit(x_mu): u*N - 0.5 = x_mu*(N-1), clamped.
+        const float u = tex_from_unit<float>(x_mu, TMU);
+        t = fminf(fmaxf(u * float(TMU) - 0.5f, 0.0f), tmax);
+    }
+    // smoothstep(-sin_h alpha, sin_h alpha, mu_s_d - cos_h) with the
+    // divisor's reciprocal from the table.
+    const float u = __builtin_amdgcn_fmed3f(fmaf(mu_s_d - lq.x, iv.w, 0.5f), 0.0f, 1.0f);
+    sm = u * u * (3.0f - 2.0f * u);
+}
+
+
+// STAGED: per sample, the block first interpolates the sample's two
+// transmittance rows (its r taps) into one row in LDS (double buffered, one
+// barrier per sample) and each texel interpolates that row in mu: all
+// texels of the (r, mu) row look up the same two LUT rows, so this replaces
+// four scattered loads per texel and sample by coalesced loads of the rows.
+// Otherwise (transmittance LUTs wider than kSsMaxStagedMu) each texel reads
+// its four taps itself.
+template <typename G, bool COMP, bool H, bool STAGED>
 __device__ __forceinline__ void single_scattering_block(const DevParams& P, const int bx, const int by, const int bz,
                                                            const float4* __restrict__ T,
                                                            const float4* __restrict__ SR,
@@ -432,70 +477,8 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
     const float rf = float(r);
     const float top2 = P.top * P.top;
     const int TMU = P.t_mu;
-    const float tmax = float(TMU - 1);
-    for (int x = tid; x < W; x += nt) {
-        TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
-        const float mu_s = float(c.mu_s), nu = float(c.nu);
-        Sum<COMP> rs[3], ms[3];
-        // The row table's entries of sample i+1 are loaded while sample i
-        // is evaluated (scalar loads, waited for at their first use).
-        float4 tv_n = s_tv[0], gq_n = s_geo[0], lq_n = s_lut[0], iv_n = s_inv[0];
-        float4 tr4_n = s_trw[0];
-        for (int i = 0; i < kMsSamples; ++i) {
-            const float4 tv = tv_n, gq = gq_n, lq = lq_n, iv = iv_n, tr4 = tr4_n;
-            const int in = i + 1 < kMsSamples ? i + 1 : i;
-            tv_n = s_tv[in];
-            gq_n = s_geo[in];
-            lq_n = s_lut[in];
-            iv_n = s_inv[in];
-            tr4_n = s_trw[in];
-            // The sample's two T rows (r taps) as one buffer: row 0 at byte 0,
-            // row 1 at `drow` (0 when both taps are the same row).
-            const int row0 = __float_as_int(tr4.x), drow = (__float_as_int(tr4.y) - row0) * 16;
-            const __amdgpu_buffer_rsrc_t tb = buffer_rsrc(T + row0, drow + TMU * 16);
-            const float r_d = gq.x, d_i = tv.w;
-            Taps tx;
-            float mu_s_d;
-            if constexpr (kPrecise) {
-                // Sun-ray distance to the top boundary without cancellation:
-                // for mu_s_d > 0 the root is (top^2 - r_d^2) / (r_d mu_s_d +
-                // sqrt(disc)), with top^2 - r_d^2 from the FP64 set-up.
-                // Reciprocals of the per-sample divisors come from FP64 too.
-                mu_s_d = __builtin_amdgcn_fmed3f(fmaf(rf, mu_s, d_i * nu) * iv.x, -1.0f, 1.0f);
-                const float rm = r_d * mu_s_d;
-                const float sq = __builtin_amdgcn_sqrtf(fmaf(rm, rm, iv.z));
-                const float d = mu_s_d > 0.0f ? iv.z * __builtin_amdgcn_rcpf(rm + sq) : sq - rm;
-                // tex_from_unit then linear_taps: texel coordinate x_mu*(N-1).
-                const float t = __builtin_amdgcn_fmed3f((d - lq.y) * iv.y * tmax, 0.0f, tmax);
-                const float fl = floorf(t);
-                tx.i0 = int(fl);
-                tx.i1 = min(tx.i0 + 1, TMU - 1);
-                tx.w = t - fl;
-            } else {
-                mu_s_d = clamp_cosine<float>((rf * mu_s + d_i * nu) / r_d);
-                // Sun transmittance T(r_d, mu_s_d) (transmittance.h:26-33, 63-74).
-                float disc = r_d * r_d * (mu_s_d * mu_s_d - 1.0f) + top2;
-                float d = clamp_distance<float>(-r_d * mu_s_d + safe_sqrt<float>(disc));
-                float x_mu = (d - lq.y) / lq.z;
-                tx = linear_taps<float>(tex_from_unit<float>(x_mu, TMU), TMU);
-            }
-            const int o0 = tx.i0 * 16, o1 = tx.i1 * 16;
-            const float3 ts = lerp3(lerp3(buffer_rgb(tb, o0), buffer_rgb(tb, o1), tx.w),
-                                    lerp3(buffer_rgb(tb, drow + o0), buffer_rgb(tb, drow + o1), tx.w),
-                                    lq.w);
-            // smoothstep(-sin_h alpha, sin_h alpha, mu_s_d - cos_h) with the
-            // divisor's reciprocal from the table.
-            const float u = __builtin_amdgcn_fmed3f(fmaf(mu_s_d - lq.x, iv.w, 0.5f), 0.0f, 1.0f);
-            const float sm = u * u * (3.0f - 2.0f * u);
-            float t0 = tv.x * (ts.x * sm), t1 = tv.y * (ts.y * sm), t2 = tv.z * (ts.z * sm);
-            rs[0].add(t0 * gq.y);
-            rs[1].add(t1 * gq.y);
-            rs[2].add(t2 * gq.y);
-            ms[0].add(t0 * gq.z);
-            ms[1].add(t1 * gq.z);
-            ms[2].add(t2 * gq.z);
-        }
-        const float dxf = s_trw[0].z;
+    const float dxf = s_trw[0].z;
+    auto store = [&](int x, const Sum<COMP>* rs, const Sum<COMP>* ms) {
         float3 ray = make_float3(rs[0].value() * dxf * P.solar[0] * P.rayleigh_scattering[0],
                                  rs[1].value() * dxf * P.solar[1] * P.rayleigh_scattering[1],
                                  rs[2].value() * dxf * P.solar[2] * P.rayleigh_scattering[2]);
@@ -506,6 +489,104 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
         st_tex(dR, idx, make_float4(ray.x, ray.y, ray.z, 0.0f), H);
         st_tex(dM, idx, make_float4(mie.x, mie.y, mie.z, 0.0f), H);
         st_tex(S, idx, make_float4(ray.x, ray.y, ray.z, mie.x), H);
+    };
+    // One sample's contribution: transmittances and densities.
+    auto accumulate = [&](Sum<COMP>* rs, Sum<COMP>* ms, float4 tv, float4 gq, float3 ts,
+                          float sm) {
+        float t0 = tv.x * (ts.x * sm), t1 = tv.y * (ts.y * sm), t2 = tv.z * (ts.z * sm);
+        rs[0].add(t0 * gq.y);
+        rs[1].add(t1 * gq.y);
+        rs[2].add(t2 * gq.y);
+        ms[0].add(t0 * gq.z);
+        ms[1].add(t1 * gq.z);
+        ms[2].add(t2 * gq.z);
+    };
+    if constexpr (STAGED) {
+        extern __shared__ float4 s_trow[];  // [2][TMU]
+        float tmu_s[kSsTexelsPerThread], tnu[kSsTexelsPerThread];
+        Sum<COMP> rs[kSsTexelsPerThread][3], ms[kSsTexelsPerThread][3];
+#pragma unroll
+        for (int q = 0; q < kSsTexelsPerThread; ++q) {
+            const int x = tid + q * nt;
+            tmu_s[q] = 0.0f;
+            tnu[q] = 0.0f;
+            if (x < W) {
+                TexelCoord<G> c =
+                    scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
+                tmu_s[q] = float(c.mu_s);
+                tnu[q] = float(c.nu);
+            }
+        }
+        // Sample i's two T rows, lerped in r with the row weight, into dst.
+        auto stage = [&](int i, float4* dst) {
+            const float4 tr4 = s_trw[i];
+            const float w = s_lut[i].w;
+            const float4* t0 = T + __float_as_int(tr4.x);
+            const float4* t1 = T + __float_as_int(tr4.y);
+            for (int j = tid; j < TMU; j += nt) {
+                const float4 a = ldg4(t0 + j), b = ldg4(t1 + j);
+                dst[j] = make_float4(fmaf(w, b.x - a.x, a.x), fmaf(w, b.y - a.y, a.y),
+                                     fmaf(w, b.z - a.z, a.z), 0.0f);
+            }
+        };
+        stage(0, s_trow);
+        __syncthreads();
+        for (int i = 0; i < kMsSamples; ++i) {
+            const float4* cur = s_trow + (i & 1) * TMU;
+            if (i + 1 < kMsSamples) stage(i + 1, s_trow + ((i + 1) & 1) * TMU);
+            const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i], iv = s_inv[i];
+#pragma unroll
+            for (int q = 0; q < kSsTexelsPerThread; ++q) {
+                if (tid + q * nt >= W) continue;
+                float t, sm;
+                ss_sun_coord<kPrecise>(rf, tmu_s[q], tnu[q], tv, gq, lq, iv, top2, TMU, t, sm);
+                const int i0 = int(t);  // t >= 0: truncation is floor
+                const int i1 = min(i0 + 1, TMU - 1);
+                const float f = t - float(i0);
+                const float4 a = cur[i0], b = cur[i1];
+                const float3 ts = make_float3(fmaf(f, b.x - a.x, a.x), fmaf(f, b.y - a.y, a.y),
+                                              fmaf(f, b.z - a.z, a.z));
+                accumulate(rs[q], ms[q], tv, gq, ts, sm);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < kSsTexelsPerThread; ++q)
+            if (tid + q * nt < W) store(tid + q * nt, rs[q], ms[q]);
+    } else {
+        for (int x = tid; x < W; x += nt) {
+            TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
+            const float mu_s = float(c.mu_s), nu = float(c.nu);
+            Sum<COMP> rs[3], ms[3];
+            // The row table's entries of sample i+1 are loaded while sample i
+            // is evaluated (scalar loads, waited for at their first use).
+            float4 tv_n = s_tv[0], gq_n = s_geo[0], lq_n = s_lut[0], iv_n = s_inv[0];
+            float4 tr4_n = s_trw[0];
+            for (int i = 0; i < kMsSamples; ++i) {
+                const float4 tv = tv_n, gq = gq_n, lq = lq_n, iv = iv_n, tr4 = tr4_n;
+                const int in = i + 1 < kMsSamples ? i + 1 : i;
+                tv_n = s_tv[in];
+                gq_n = s_geo[in];
+                lq_n = s_lut[in];
+                iv_n = s_inv[in];
+                tr4_n = s_trw[in];
+                // The sample's two T rows (r taps) as one buffer: row 0 at byte
+                // 0, row 1 at `drow` (0 when both taps are the same row).
+                const int row0 = __float_as_int(tr4.x),
+                          drow = (__float_as_int(tr4.y) - row0) * 16;
+                const __amdgpu_buffer_rsrc_t tb = buffer_rsrc(T + row0, drow + TMU * 16);
+                float t, sm;
+                ss_sun_coord<kPrecise>(rf, mu_s, nu, tv, gq, lq, iv, top2, TMU, t, sm);
+                const int i0 = int(t);
+                const int o0 = i0 * 16, o1 = min(i0 + 1, TMU - 1) * 16;
+                const float f = t - float(i0);
+                const float3 ts =
+                    lerp3(lerp3(buffer_rgb(tb, o0), buffer_rgb(tb, o1), f),
+                          lerp3(buffer_rgb(tb, drow + o0), buffer_rgb(tb, drow + o1), f), lq.w);
+                accumulate(rs, ms, tv, gq, ts, sm);
+            }
+            store(x, rs, ms);
+        }
     }
 }
 
@@ -1349,7 +1430,7 @@ __device__ __forceinline__ unsigned logical_block(bool remap) {
     return remap ? xcd_remap(L, gridDim.x * gridDim.y * gridDim.z) : L;
 }
 
-template <typename G, bool COMP, bool H>
+template <typename G, bool COMP, bool H, bool STAGED>
 __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const float4* __restrict__ T,
                                                            const float4* __restrict__ SR,
                                                            float4* __restrict__ dR,
@@ -1357,9 +1438,9 @@ __global__ __launch_bounds__(256) void k_single_scattering(DevParams P, const fl
                                                            float4* __restrict__ S,
                                                            unsigned remap) {
     const unsigned b = logical_block(remap);
-    single_scattering_block<G, COMP, H>(P, b % P.s_mu, b / P.s_mu, 0, T, SR, dR, dM, S);
+    single_scattering_block<G, COMP, H, STAGED>(P, b % P.s_mu, b / P.s_mu, 0, T, SR, dR, dM, S);
 }
-template <typename G, bool COMP, bool H>
+template <typename G, bool COMP, bool H, bool STAGED>
 __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __restrict__ Ps,
                                                             const LutPtrs* __restrict__ L,
                                                             const uint32_t* __restrict__ tiles, unsigned ntiles) {
@@ -1372,8 +1453,8 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs l = lut_ptrs(L, atm);
-        single_scattering_block<G, COMP, H>(P, local % P.s_mu, local / P.s_mu, 0, l.T, l.SR, l.dR,
-                                         l.dM, l.S);
+        single_scattering_block<G, COMP, H, STAGED>(P, local % P.s_mu, local / P.s_mu, 0, l.T,
+                                                    l.SR, l.dR, l.dM, l.S);
     }
 }
 
@@ -1555,10 +1636,19 @@ static hipError_t launch_single_scattering_h(const DevParams& P, const float4* T
     const unsigned threads = tuned_threads(P, kPassSingle, 2, tu);
     const unsigned remap = (tu.remap >> kPassSingle) & 1u;
     dim3 grid(P.s_mu, P.s_r);
-    if (P.flags & 1)
-        k_single_scattering<double, false, H><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
-    else
-        k_single_scattering<float, false, H><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
+    if (ss_staged(P)) {
+        const size_t lds = ss_staged_lds_bytes(P.t_mu);
+        if (P.flags & 1)
+            k_single_scattering<double, false, H, true>
+                <<<grid, threads, lds, s>>>(P, T, SR, dR, dM, S, remap);
+        else
+            k_single_scattering<float, false, H, true>
+                <<<grid, threads, lds, s>>>(P, T, SR, dR, dM, S, remap);
+    } else if (P.flags & 1) {
+        k_single_scattering<double, false, H, false><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
+    } else {
+        k_single_scattering<float, false, H, false><<<grid, threads, 0, s>>>(P, T, SR, dR, dM, S, remap);
+    }
     return hipGetLastError();
 }
 hipError_t launch_single_scattering(const DevParams& P, const float4* T, const float4* SR,
@@ -1702,8 +1792,8 @@ bool tuning_valid(const DevParams& P, int pass, int order, unsigned threads) {
                W <= threads * unsigned(kMsTexelsPerThread);
     if (threads > 256) return false;
     switch (pass) {
-        case kPassSingle:
-            return true;
+        case kPassSingle:  // the staged kernel holds at most 4 texels per thread
+            return !ss_staged(P) || W <= threads * unsigned(kSsTexelsPerThread);
         case kPassDensity:
             return true;
         case kPassIndirect:
@@ -1725,7 +1815,8 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
         case kPassDirect:
             return {cdiv(P.i_mus, 64) * unsigned(P.i_r), 64, 0};
         case kPassSingle:
-            return {unsigned(P.s_mu) * unsigned(P.s_r), threads, 0};
+            return {unsigned(P.s_mu) * unsigned(P.s_r), threads,
+                    ss_staged(P) ? ss_staged_lds_bytes(P.t_mu) : 0};
         case kPassClear:
             return {cdiv(ni, 256), 256, 0};
         case kPassGeometry:
@@ -1767,11 +1858,15 @@ static hipError_t launch_batch_h(int pass, int order, bool precise, const DevPar
             else
                 kb_direct_irradiance<float><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
-        case kPassSingle:
-            if (precise)
-                kb_single_scattering<double, false, H><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+        case kPassSingle:  // tpt: 1 for the staged kernel (lds > 0), 0 otherwise
+            if (precise && tpt)
+                kb_single_scattering<double, false, H, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+            else if (precise)
+                kb_single_scattering<double, false, H, false><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+            else if (tpt)
+                kb_single_scattering<float, false, H, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else
-                kb_single_scattering<float, false, H><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
+                kb_single_scattering<float, false, H, false><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
         case kPassClear:
             kb_clear_irradiance<<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
@@ -1842,8 +1937,12 @@ static unsigned batch_resident_blocks_h(int pass, int order, bool precise, unsig
                          : (const void*)kb_direct_irradiance<float>;
             break;
         case kPassSingle:
-            fn = precise ? (const void*)kb_single_scattering<double, false, H>
-                         : (const void*)kb_single_scattering<float, false, H>;
+            if (tpt)
+                fn = precise ? (const void*)kb_single_scattering<double, false, H, true>
+                             : (const void*)kb_single_scattering<float, false, H, true>;
+            else
+                fn = precise ? (const void*)kb_single_scattering<double, false, H, false>
+                             : (const void*)kb_single_scattering<float, false, H, false>;
             break;
         case kPassClear:
             fn = (const void*)kb_clear_irradiance;
