@@ -177,3 +177,18 @@ def test_mixed_workload_matches_committed_fixture():
     with open(path) as f:
         fixed = [Parameters.from_dict(d) for d in json.load(f)]
     assert fixed == mixed_batch(256, 4, precise=True)
+
+
+def test_large_fixture_members_are_mixed_set_members():
+    """tests/golden/mixed_large_fp64.npz holds members of the committed
+    256-atmosphere set: its parameters equal mixed256_seed4.json at
+    mix_index, three at "default" and three at "double" resolution."""
+    import json
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    g = np.load(os.path.join(here, "mixed_large_fp64.npz"))
+    mix = json.load(open(os.path.join(here, "mixed256_seed4.json")))
+    ps = json.loads(str(g["params"]))
+    assert [mix[int(i)] for i in g["mix_index"]] == ps
+    assert sorted(p["scattering_nu_size"] for p in ps) == [8, 8, 8, 16, 16, 16]

@@ -121,3 +121,53 @@ def test_mixed_half_members_vs_fp64_fixture(builder):
     assert worst["T"] < 1e-5, worst
     assert worst["E"] < 1e-3, worst
     assert worst["S"] < 1e-3, worst
+
+
+def test_mixed_members_all_resolutions_vs_fp64_fixture(builder):
+    """The precision qualifier on the sizes that dominate the mixed set's
+    time: six members of the seeded 256-atmosphere set itself at the
+    "default" (32x128x32x8) and "double" (64x256x64x16) resolutions
+    (tests/golden/make_golden_mixed_large.py), built as ONE precision-mode
+    batch together with the eight "half" fixtures, each against its FP64
+    oracle LUTs: max relative error <= 1e-3 (1e-5 for transmittance), floor
+    1e-6 x max per channel."""
+    import json
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    members = []
+    for name in ("mixed_half_fp64.npz", "mixed_large_fp64.npz"):
+        path = os.path.join(here, name)
+        if not os.path.exists(path):
+            pytest.skip(f"{name} not generated")
+        g = np.load(path)
+        for i, d in enumerate(json.loads(str(g["params"]))):
+            idx = g[f"S_idx{i}"] if f"S_idx{i}" in g else g["S_idx"]
+            members.append((Parameters.from_dict(d), g[f"T{i}"], g[f"E{i}"], g[f"S{i}"], idx))
+    pend = Atmosphere.build_batch(builder, None, [m[0] for m in members])
+    worst = {}
+    for (p, T64, E64, S64, idx), pa in zip(members, pend):
+        pa.wait()
+        a = pa.atmosphere()
+        got = {"T": (a.download("transmittance")[..., :3], T64),
+               "E": (a.download("irradiance")[..., :3], E64),
+               "S": (a.download("scattering").reshape(-1, 4)[idx], S64)}
+        cls = {4: "half", 8: "default", 16: "double"}[p.scattering_nu_size]
+        errs = {}
+        for key, (x, y) in got.items():
+            x = x.astype(np.float64)
+            y = y.astype(np.float64)
+            scale = np.abs(y).reshape(-1, y.shape[-1]).max(axis=0)
+            errs[key] = float((np.abs(x - y) / np.maximum(np.abs(y), 1e-6 * scale)).max())
+            w = worst.setdefault(cls, {})
+            w[key] = max(w.get(key, 0.0), errs[key])
+        print(f"\n{cls} member (order {p.order}, bottom {p.bottom_radius:.0f} km): {errs}",
+              flush=True)
+    for pa in pend:
+        pa.close()
+    print(f"\nworst by class: {worst}", flush=True)
+    assert set(worst) == {"half", "default", "double"}
+    for cls, w in worst.items():
+        assert w["T"] < 1e-5, (cls, w)
+        assert w["E"] < 1e-3, (cls, w)
+        assert w["S"] < 1e-3, (cls, w)

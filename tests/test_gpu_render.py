@@ -276,3 +276,58 @@ def test_empty_inputs(builder, atm):
     r.sun_and_sky_irradiance(None, a, one, one, one, 0, 0, out, out)
     synchronize()
     assert Atmosphere.build_batch(builder, None, []) == []
+
+
+def test_default_size_frames_vs_oracle(builder):
+    """The bench's render workload on the LUT set it is timed on: the
+    default-size Earth set (scattering 32x128x32x8, precision mode) and
+    1920x1080 frames from fb_render_sky_frames (cameras at U[0, 10] km, sun
+    zeniths up to 108 degrees: the per-frame LDS slice of 22.5 KB), 20,000
+    sampled pixels per frame against the FP32 restatement of the same
+    lookups.  Also views exactly opposite the sun through the per-ray path,
+    where |nu| rounds past 1."""
+    p = Parameters().replace(flags=1)
+    pa = Atmosphere.build(builder, None, p)
+    pa.wait()
+    a = pa.assert_ready()
+    T = a.download("transmittance").astype(np.float64)
+    S = a.download("scattering").astype(np.float64)
+    frames, px = 4, 1920 * 1080
+    alt = np.array([0.0, 2.5, 7.0, 10.0], np.float32)
+    zen = np.radians(np.array([5.0, 60.0, 95.0, 108.0])).astype(np.float32)
+    r = Renderer(builder)
+    d_alt, d_zen = DeviceBuffer.from_array(alt), DeviceBuffer.from_array(zen)
+    rad, tr = DeviceBuffer(frames * px * 16), DeviceBuffer(frames * px * 16)
+    r.frames(None, a, d_alt, d_zen, frames, px, 2, rad, tr)
+    synchronize()
+    g_rad = rad.to_array(np.float32, (frames, px, 4))
+    g_tr = tr.to_array(np.float32, (frames, px, 4))
+    rng = np.random.default_rng(31)
+    worst = 0.0
+    for f in range(frames):
+        idx = np.sort(rng.choice(px, 20000, replace=False))
+        views = np.stack([ray_direction(2, f, int(i)) for i in idx])
+        cams = np.array([[0.0, 0.0, np.float32(6360.0) + alt[f]]], np.float32)
+        sun = np.array([[math.sin(zen[f]), 0.0, math.cos(zen[f])]], np.float32)
+        o_rad, o_tr = Oracle("f32").sky_rays(p, T, S, cams, views, sun)
+        e_rad = rel_err(g_rad[f, idx], o_rad, 1e-6).max()
+        e_tr = rel_err(g_tr[f, idx], o_tr, 1e-6).max()
+        worst = max(worst, e_rad, e_tr)
+        assert e_rad < 1e-4 and e_tr < 1e-4, (f, e_rad, e_tr)
+    print(f"\ndefault-size frames vs FP32 restatement: max rel err {worst:.3e}", flush=True)
+    # views exactly opposite the sun (anti-solar point), per-ray path
+    n = 256
+    zs = np.radians(np.linspace(0.0, 108.0, n))
+    suns = np.stack([np.sin(zs), np.zeros(n), np.cos(zs)], axis=1).astype(np.float32)
+    views = (-suns).astype(np.float32)
+    cams = np.stack([np.zeros(n), np.zeros(n), 6360.0 + np.linspace(0.0, 10.0, n)],
+                    axis=1).astype(np.float32)
+    bufs = [DeviceBuffer.from_array(x) for x in (cams, views, suns)]
+    rr, rt = DeviceBuffer(n * 16), DeviceBuffer(n * 16)
+    r.rays(None, a, bufs[0], 1, bufs[1], bufs[2], 1, n, rr, rt)
+    synchronize()
+    o_rad, _ = Oracle("f32").sky_rays(p, T, S, cams, views, suns)
+    g = rr.to_array(np.float32, (n, 4))
+    assert np.all(np.isfinite(g))
+    assert rel_err(g, o_rad, 1e-6).max() < 1e-4
+    a.close()
