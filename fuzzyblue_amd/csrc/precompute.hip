@@ -689,6 +689,7 @@ __device__ __forceinline__ PairEntry make_pair_entry(float4 rv, float4 rw, float
 struct DenTexel {
     float omx, omz;          // omega = (omx, 0, omz)
     float osx, osy, osz;     // omega_s
+    float osxh, osyh;        // omega_s.x, .y x (NU-1)/2
     float g1, g2;            // 1 + g^2, -2g
     float hnu, hE;           // (NU-1)/2, (IRR_MUS-1)/2
     float kmax, emax;        // NU-1, IRR_MUS-1
@@ -697,22 +698,27 @@ struct DenTexel {
 // Per-(texel, theta) constants.
 struct DenRow {
     float C, omzc;           // cos(theta)*omega_s.z, cos(theta)*omega.z
+    float Ch;                // (C + 1) x (NU-1)/2
     const ProfEntry* pr;     // nu profile of multiple scattering (order n-1 >= 2)
     const PairEntry* pp;     // nu profiles of single Rayleigh and Mie (order 1)
     const float4* dir;       // c_den_dir row: (cos phi sin t, sin phi sin t, cos t, dw)
     float3 gv;               // T_g * albedo / pi (ground rows)
-    float Ag, Bg, Cg;        // ground normal . omega_s = cx*Ag + sy*Bg + Cg
+    float Ag, Bg, Cg;        // ground normal . omega_s = cx*Ag + sy*Bg + Cg, each
+                             // x (IRR_MUS-1)/2, Cg + 1 x (IRR_MUS-1)/2
 };
 
-// Linear filtering at texel coordinate t, clamped to [0, kmax] first.  For
-// t in range this is exactly the CLAMP_TO_EDGE result of prof_lerp (at
-// t = kmax the delta has weight 0); a coordinate that rounding put just
-// outside the range (|nu| a few ulp above 1) reads the end value.  With
-// t >= 0, truncation is floor, so the slot and weight take one conversion
-// and one fract.
+// Linear filtering at texel coordinate t >= 0 after a clamp at 0.  For t
+// in [0, kmax] this is exactly the CLAMP_TO_EDGE result of prof_lerp (at
+// t = kmax the delta has weight 0).  Coordinates come from unit vectors, so
+// rounding puts t at most a few ulp outside [0, kmax]: below 0 it reads the
+// first value; just above kmax, int(t) is still kmax (the last entry) and
+// the weight of its delta a few ulp.  With t >= 0, truncation is floor, so
+// the slot and weight take one conversion and one fract.  (A max: v_med3
+// issues at half the rate of the other VALU operations.)
 __device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof, float t,
                                               float kmax) {
-    t = __builtin_amdgcn_fmed3f(t, 0.0f, kmax);
+    (void)kmax;
+    t = fmaxf(t, 0.0f);
     const int k = int(t);
     const float f = __builtin_amdgcn_fractf(t);
     const ProfEntry* e = prof + (k + 1);
@@ -723,7 +729,8 @@ __device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof
 // The same for a Rayleigh / Mie pair of profiles.
 __device__ __forceinline__ void pair_lerp_i(const PairEntry* __restrict__ prof, float t,
                                             float kmax, float3& R, float3& M) {
-    t = __builtin_amdgcn_fmed3f(t, 0.0f, kmax);
+    (void)kmax;
+    t = fmaxf(t, 0.0f);
     const int k = int(t);
     const float f = __builtin_amdgcn_fractf(t);
     const PairEntry* e = prof + (k + 1);
@@ -748,11 +755,11 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
         const float nu2 = fmaf(cx, dt.omx, row.omzc);
         const float q = fmaf(nu2, nu2, 1.0f);
         const float qw = q * rsqrt_cubed(fmaf(dt.g2, nu2, dt.g1));
-        const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
-        const float nua = base + sb, nub = base - sb;
-        const float ta = fmaf(nua, dt.hnu, dt.hnu), tb = fmaf(nub, dt.hnu, dt.hnu);
         float3 La, Lb;
         if (SINGLE) {
+            const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
+            const float nua = base + sb, nub = base - sb;
+            const float ta = fmaf(nua, dt.hnu, dt.hnu), tb = fmaf(nub, dt.hnu, dt.hnu);
             // Profiles are pre-scaled by kR (rayleigh) and kM (mie).
             float3 Ma, Mb;
             pair_lerp_i(row.pp, ta, dt.kmax, La, Ma);
@@ -765,13 +772,16 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
             Lb = make_float3(qb * fmaf(Mb.x, wb, Lb.x), qb * fmaf(Mb.y, wb, Lb.y),
                              qb * fmaf(Mb.z, wb, Lb.z));
         } else {
-            La = prof_lerp_i(row.pr, ta, dt.kmax);
-            Lb = prof_lerp_i(row.pr, tb, dt.kmax);
+            // texel coordinate (nu + 1) (NU-1)/2 with the scale folded into
+            // the per-texel and per-row terms
+            const float base = fmaf(cx, dt.osxh, row.Ch), sb = sy * dt.osyh;
+            La = prof_lerp_i(row.pr, base + sb, dt.kmax);
+            Lb = prof_lerp_i(row.pr, base - sb, dt.kmax);
         }
         if (GROUND) {
             const float gb = fmaf(cx, row.Ag, row.Cg), gs = sy * row.Bg;
-            const float3 Ea = prof_lerp_i(dt.erow, fmaf(gb + gs, dt.hE, dt.hE), dt.emax);
-            const float3 Eb = prof_lerp_i(dt.erow, fmaf(gb - gs, dt.hE, dt.hE), dt.emax);
+            const float3 Ea = prof_lerp_i(dt.erow, gb + gs, dt.emax);
+            const float3 Eb = prof_lerp_i(dt.erow, gb - gs, dt.emax);
             La = make_float3(fmaf(row.gv.x, Ea.x, La.x), fmaf(row.gv.y, Ea.y, La.y),
                              fmaf(row.gv.z, Ea.z, La.z));
             Lb = make_float3(fmaf(row.gv.x, Eb.x, Lb.x), fmaf(row.gv.y, Eb.y, Lb.y),
@@ -962,6 +972,8 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     dt.osx = fminf(fmaxf(float(sdx), -1.0f), 1.0f);
     dt.osy = fminf(fmaxf(float(sdy), -1.0f), 1.0f);
     dt.osz = float(c.mu_s);
+    dt.osxh = dt.osx * (0.5f * float(NU - 1));
+    dt.osyh = dt.osy * (0.5f * float(NU - 1));
     const float rf = float(c.r);
     const float alt = float(c.r - G(P.bottom));
     const float rho_r = profile_density(P.prof[0], alt);
@@ -990,6 +1002,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         const float ct = d0.z, dom = d0.w;
         DenRow row;
         row.C = ct * dt.osz;
+        row.Ch = fmaf(row.C, dt.hnu, dt.hnu);
         row.omzc = ct * dt.omz;
         row.pr = s_prof + l * nent;
         row.pp = s_pair + l * nent;
@@ -999,9 +1012,9 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
             const float inv = s_ginv[l];
             const float dgi = gv.w * inv;
             row.gv = make_float3(gv.x, gv.y, gv.z);
-            row.Ag = dgi * dt.osx;
-            row.Bg = dgi * dt.osy;
-            row.Cg = (rf + ct * gv.w) * inv * dt.osz;
+            row.Ag = dgi * dt.osx * dt.hE;
+            row.Bg = dgi * dt.osy * dt.hE;
+            row.Cg = fmaf((rf + ct * gv.w) * inv * dt.osz, dt.hE, dt.hE);
             density_row<SINGLE, true>(dt, row, aR, aM);
         } else {
             density_row<SINGLE, false>(dt, row, aR, aM);
