@@ -212,6 +212,31 @@ class Atmosphere:
             out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 
+    def save(self, which, path: str, fp16: bool = False):
+        """Write one LUT as an OpenEXR image laid out like the reference's
+        dump (examples/dump.rs:331-367); fp16 writes the scattering LUT as
+        half (the dump's format)."""
+        check(load().fb_atmosphere_save(
+            self.handle, _lut_id(which), abi.FB_FORMAT_F16 if fp16 else abi.FB_FORMAT_F32,
+            str(path).encode()))
+
+    def save_all(self, prefix: str, fp16: bool = False) -> List[str]:
+        """prefix + "transmittance.exr", "irradiance.exr", "scattering.exr"."""
+        paths = [f"{prefix}{n}.exr" for n in ("transmittance", "irradiance", "scattering")]
+        for name, path in zip(("transmittance", "irradiance", "scattering"), paths):
+            self.save(name, path, fp16)
+        return paths
+
+    @staticmethod
+    def load(builder: Builder, transmittance: str, irradiance: str,
+             scattering: str) -> "Atmosphere":
+        """A finished atmosphere from files written by save()."""
+        out = C.c_void_p()
+        check(load().fb_atmosphere_load(builder.handle, str(transmittance).encode(),
+                                        str(irradiance).encode(), str(scattering).encode(),
+                                        C.byref(out)))
+        return Atmosphere(out, builder, owned=True)
+
     def close(self):
         if self.handle and self._owned:
             load().fb_atmosphere_destroy(self.handle)

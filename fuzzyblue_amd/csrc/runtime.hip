@@ -218,6 +218,32 @@ bool atmosphere_idle(fb_atmosphere* a) {
     return true;
 }
 
+// A finished atmosphere's outputs (T, E, S) and completion event, without
+// the scratch of a build: for LUT sets read from files.
+fb_status alloc_atmosphere(fb_builder* b, const fb_parameters& p, fb_atmosphere** out) {
+    Sizes s = sizes_of(p);
+    auto* a = new fb_atmosphere();
+    a->builder = b;
+    builder_retain(b);
+    a->params = p;
+    a->dev = make_dev_params(p);
+    fb_status st = alloc_buf(b, s.nt, &a->T);
+    if (st == FB_OK) st = alloc_buf(b, s.ni, &a->E);
+    if (st == FB_OK) st = alloc_buf(b, s.ns, &a->S, s_texel_bytes(p));
+    if (st == FB_OK) {
+        hipError_t e = hipEventCreateWithFlags(&a->ready, hipEventDisableTiming);
+        if (e != hipSuccess) st = hip_fail(e, "hipEventCreate");
+    }
+    if (st != FB_OK) {
+        free_atmosphere_buffers(a);
+        delete a;
+        builder_release(b);
+        return st;
+    }
+    *out = a;
+    return FB_OK;
+}
+
 fb_status alloc_luts(fb_builder* b, const fb_parameters& p, fb_atmosphere** out_a,
                      fb_pending_atmosphere** out_pa) {
     Sizes s = sizes_of(p);

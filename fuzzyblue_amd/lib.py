@@ -84,6 +84,8 @@ def load():
         "fb_atmosphere_parameters": (S, [V, P]),
         "fb_atmosphere_download": (S, [V, S, C.c_uint32, V, C.c_size_t]),
         "fb_atmosphere_destroy": (None, [V]),
+        "fb_atmosphere_save": (S, [V, S, C.c_uint32, C.c_char_p]),
+        "fb_atmosphere_load": (S, [V, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(V)]),
         "fb_render_sky": (
             S,
             [V, V, V, C.POINTER(abi.fb_draw_parameters), V, C.c_uint32, C.c_uint32, V, V,

@@ -344,6 +344,27 @@ fb_status fb_atmosphere_parameters(const fb_atmosphere *a, fb_parameters *out);
 fb_status fb_atmosphere_download(const fb_atmosphere *a, fb_lut which,
                                  uint32_t format, void *host, size_t bytes);
 
+/* LUT files: the images examples/dump.rs:331-367 writes with the openexr
+ * crate, one per LUT — width x height, channels "R","G","B","A" for a 2D
+ * LUT and "{z}.R" .. "{z}.A" for depth slice z of the scattering LUT, the
+ * linear layout of examples/dump.rs:175-193 — as uncompressed single-part
+ * scanline OpenEXR files.  Transmittance and irradiance are 32-bit float;
+ * the scattering LUT is 16-bit half with FB_FORMAT_F16 (the dump's format)
+ * or 32-bit float with FB_FORMAT_F32.  Two header attributes,
+ * "fuzzyblueLut" and "fuzzyblueParameters" (the fb_parameters bytes, host
+ * byte order), let fb_atmosphere_load rebuild the set.  Save synchronises
+ * with the atmosphere's completion event. */
+fb_status fb_atmosphere_save(const fb_atmosphere *a, fb_lut which,
+                             uint32_t format, const char *path);
+/* A finished atmosphere from three files written by fb_atmosphere_save
+ * (same atmosphere, any scattering format): the LUTs are uploaded into the
+ * builder's pool in the saved parameters' storage format (binary16 widens
+ * exactly, float32 rounds to nearest even) and the set is ready on return.
+ * FB_EINVAL for files of another layout, size or atmosphere. */
+fb_status fb_atmosphere_load(fb_builder *b, const char *transmittance_path,
+                             const char *irradiance_path,
+                             const char *scattering_path, fb_atmosphere **out);
+
 /* Drop for Atmosphere (src/precompute.rs:1059-1073). */
 void fb_atmosphere_destroy(fb_atmosphere *a);
 
