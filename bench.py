@@ -427,6 +427,10 @@ def measure_ceilings():
            "valu_mul_sgpr_wave_instr_per_s": run(13, 64) / 64.0,
            "valu_max_mul_wave_instr_per_s": run(14, 64) / 64.0,
            "valu_add_fract_wave_instr_per_s": run(15, 64) / 64.0,
+           "valu_lshl_add_vgpr_wave_instr_per_s": run(16, 64) / 64.0,
+           "valu_mul_sgpr_alternating_wave_instr_per_s": run(17, 64) / 64.0,
+           "valu_rsq_wave_instr_per_s": run(18, 64) / 64.0,
+           "valu_mul_cvt_cvt_wave_instr_per_s": run(19, 64) / 64.0,
            "probe": "fuzzyblue_amd/csrc/ceilings.hip, 8192 blocks x 256 threads, 5 launches"}
     for d in (1, 8, 16, 64):
         res[f"lds_b128_b64_tbps_distinct{d}"] = run(1, 64, d) / 1e12

@@ -40,7 +40,9 @@ __global__ __launch_bounds__(256) void k_fma(float* __restrict__ out, int iters,
 // VGPR operands (the VOP3 form: the result replaces a multiplicand),
 // 4 v_med3_f32, 5 v_lshl_add_u32, 6 v_fma_f32 with a scalar (SGPR)
 // multiplicand, 7 v_fma_f32 with the inline constant 1.0 as addend, 8
-// v_mul_f32 with a scalar operand, 9 v_max_f32, 10 v_fract_f32.
+// v_mul_f32 with a scalar operand, 9 v_max_f32, 10 v_fract_f32, 11
+// v_lshl_add_u32 with VGPR operands only, 12 v_mul_f32 with a scalar operand
+// alternating with v_mul_f32 on VGPRs, 13 v_rsq_f32, 14 v_cvt_i32_f32.
 template <int OP>
 __global__ __launch_bounds__(256) void k_valu(float* __restrict__ out, int iters, int shift,
                                               float sa) {
@@ -68,6 +70,12 @@ __global__ __launch_bounds__(256) void k_valu(float* __restrict__ out, int iters
                 if (OP == 8) x[k] = x[k] * sa;
                 if (OP == 9) x[k] = fmaxf(x[k] * 0.5f, a[k]);
                 if (OP == 10) x[k] = __builtin_amdgcn_fractf(x[k] + a[k]);
+                if (OP == 11)
+                    x[k] = __int_as_float((__float_as_int(x[k]) << (__float_as_int(c[k]) & 7)) +
+                                          __float_as_int(a[k]));
+                if (OP == 12) x[k] = (k & 1) ? x[k] * sa : x[k] * a[k];
+                if (OP == 13) x[k] = __builtin_amdgcn_rsqf(x[k]);
+                if (OP == 14) x[k] = float(int(x[k] * a[k]));
             }
         }
     }
@@ -153,14 +161,14 @@ extern "C" {
 
 // Runs `kind` (0: v_fma_f32 with a scalar operand; 1..3: LDS modes 0..2;
 // 4..6: v_fmac_f32, v_add_f32, v_mul_f32 chains; 7: split-layout b128 +
-// b64 entry reads; 8..15: k_valu operations 3..10) with `blocks` x 256 threads
+// b64 entry reads; 8..19: k_valu operations 3..14) with `blocks` x 256 threads
 // and `iters` outer iterations, `reps` times back to back on the default
 // stream, and returns the mean milliseconds per launch in *ms.  `out` holds
 // blocks * 256 floats.  Returns a hipError_t value (0 on success).
 int fb_ceiling_run(int kind, int blocks, int iters, int distinct, int reps, float* out,
                    float* ms) {
     if (blocks <= 0 || iters <= 0 || reps <= 0 || !out || !ms || distinct < 1 ||
-        distinct > 64 || kind < 0 || kind > 15)
+        distinct > 64 || kind < 0 || kind > 19)
         return int(hipErrorInvalidValue);
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
@@ -183,6 +191,10 @@ int fb_ceiling_run(int kind, int blocks, int iters, int distinct, int reps, floa
             case 13: k_valu<8><<<blocks, 256>>>(out, iters, 1, 0.999f); break;
             case 14: k_valu<9><<<blocks, 256>>>(out, iters, 1, 0.999f); break;
             case 15: k_valu<10><<<blocks, 256>>>(out, iters, 1, 0.999f); break;
+            case 16: k_valu<11><<<blocks, 256>>>(out, iters, 1, 0.999f); break;
+            case 17: k_valu<12><<<blocks, 256>>>(out, iters, 1, 0.999f); break;
+            case 18: k_valu<13><<<blocks, 256>>>(out, iters, 1, 0.999f); break;
+            case 19: k_valu<14><<<blocks, 256>>>(out, iters, 1, 0.999f); break;
             case 7: k_lds_split<<<blocks, 256>>>(out, iters, distinct); break;
         }
     };
@@ -207,7 +219,7 @@ double fb_ceiling_work(int kind, int blocks, int iters) {
     const double lanes = double(blocks) * 256.0 * double(iters) * kUnroll;
     switch (kind) {
         case 0: case 4: case 5: case 6: case 8: case 9: case 10: case 11: case 12: case 13:
-        case 14: case 15: return lanes * 8.0;
+        case 14: case 15: case 16: case 17: case 18: case 19: return lanes * 8.0;
         case 1: case 7: return lanes * 24.0;
         case 2: return lanes * 16.0;
         default: return lanes * 8.0;
