@@ -2,9 +2,11 @@
 // precompute kernels are held against (libfb_ceilings.so, loaded by the
 // benchmark only; the product library does not contain them):
 //
-//  * the issue rate of non-packed v_fma_f32 (the library is compiled with
-//    packed FP32 off, so the published 157.3 TFLOP/s FP32 figure, which
-//    counts packed FMAs, is not its ceiling);
+//  * VALU issue rates of single operations in the operand forms the
+//    kernels use (VGPR, SGPR or inline-constant sources): one wave64 FP32
+//    instruction per two cycles per SIMD is 1.229e12 instructions/s over
+//    256 CUs at 2.4 GHz (the published 157.3 TFLOP/s in FMAs), and some
+//    forms issue at half or a quarter of that;
 //  * the LDS read rate of ds_read_b128 / ds_read_b64 in the access pattern
 //    of the scattering-density quadrature: each lane reads a 24-byte
 //    (value, delta) entry — one b128 and one b64 — of a 32-byte-strided
