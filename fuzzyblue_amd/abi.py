@@ -151,6 +151,24 @@ class fb_lut_view(C.Structure):
     ]
 
 
+class fb_lut_export(C.Structure):
+    _fields_ = [
+        ("fd", C.c_int),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("depth", C.c_uint32),
+        ("texel_bytes", C.c_uint32),
+        ("vk_format", C.c_uint32),
+        ("bytes", C.c_size_t),
+        ("alloc_bytes", C.c_size_t),
+        ("offset", C.c_size_t),
+        ("row_pitch", C.c_size_t),
+        ("slice_pitch", C.c_size_t),
+        ("device_ptr", C.c_void_p),
+        ("opaque", C.c_void_p),
+    ]
+
+
 class fb_draw_parameters(C.Structure):
     _fields_ = [
         ("inverse_viewproj", (C.c_float * 4) * 4),

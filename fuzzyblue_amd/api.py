@@ -212,6 +212,18 @@ class Atmosphere:
             out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 
+    def export(self, which, stream=None) -> abi.fb_lut_export:
+        """A copy of the LUT in memory exportable as a dma-buf descriptor
+        (fb_atmosphere_export); release it with release_export()."""
+        e = abi.fb_lut_export()
+        check(load().fb_atmosphere_export(self.handle, _lut_id(which), _stream_handle(stream),
+                                          C.byref(e)))
+        return e
+
+    @staticmethod
+    def release_export(e: abi.fb_lut_export):
+        load().fb_lut_export_release(C.byref(e))
+
     def save(self, which, path: str, fp16: bool = False):
         """Write one LUT as an OpenEXR image laid out like the reference's
         dump (examples/dump.rs:331-367); fp16 writes the scattering LUT as

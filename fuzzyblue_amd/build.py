@@ -23,7 +23,7 @@ CEILINGS_SOURCES = ["ceilings.hip"]
 ARCH = os.environ.get("FB_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["precompute.hip", "render.hip", "runtime.hip", "batch.hip", "server.hip",
-           "tune.hip", "noalloc.hip", "lutio.hip"]
+           "tune.hip", "noalloc.hip", "lutio.hip", "interop.hip"]
 
 COMMON = [
     f"--offload-arch={ARCH}",

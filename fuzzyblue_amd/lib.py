@@ -86,6 +86,8 @@ def load():
         "fb_atmosphere_destroy": (None, [V]),
         "fb_atmosphere_save": (S, [V, S, C.c_uint32, C.c_char_p]),
         "fb_atmosphere_load": (S, [V, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(V)]),
+        "fb_atmosphere_export": (S, [V, S, V, C.POINTER(abi.fb_lut_export)]),
+        "fb_lut_export_release": (None, [C.POINTER(abi.fb_lut_export)]),
         "fb_render_sky": (
             S,
             [V, V, V, C.POINTER(abi.fb_draw_parameters), V, C.c_uint32, C.c_uint32, V, V,

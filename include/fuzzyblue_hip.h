@@ -365,6 +365,43 @@ fb_status fb_atmosphere_load(fb_builder *b, const char *transmittance_path,
                              const char *irradiance_path,
                              const char *scattering_path, fb_atmosphere **out);
 
+/* A LUT shared with another API as a Linux dma-buf descriptor, in place of
+ * the reference's images that the application samples itself
+ * (src/precompute.rs:1167-1228: created with the caller's extra usage
+ * flags, Parameters::usage, and handed out by Atmosphere::{transmittance,
+ * irradiance,scattering}, src/precompute.rs:2075-2101).  The LUT is copied
+ * (device to device, ordered after the atmosphere's completion on `stream`;
+ * the call returns when the copy is done) into an allocation of its own made
+ * exportable as a POSIX file descriptor: bytes from offset, texels in the
+ * linear x, y, z order with the pitches given, in vk_format (97
+ * R16G16B16A16_SFLOAT, 109 R32G32B32A32_SFLOAT), zero padding up to
+ * alloc_bytes.  Import fd with VK_EXT_external_memory_dma_buf
+ * (VkImportMemoryFdInfoKHR, allocationSize alloc_bytes; Vulkan takes
+ * ownership of fd on success) or hipImportExternalMemory.  device_ptr is the
+ * library's own mapping of the same memory.  The export does not depend on
+ * the atmosphere afterwards. */
+typedef struct fb_lut_export {
+    int fd;               /* owned by the caller until passed on */
+    uint32_t width;
+    uint32_t height;
+    uint32_t depth;       /* 1 for 2D LUTs */
+    uint32_t texel_bytes; /* 16, or 8 for binary16 storage */
+    uint32_t vk_format;
+    size_t bytes;         /* texels * texel_bytes */
+    size_t alloc_bytes;   /* size of the shared allocation */
+    size_t offset;        /* of the first texel in the shared allocation */
+    size_t row_pitch;
+    size_t slice_pitch;
+    void *device_ptr;
+    void *opaque;         /* the library's handle, for fb_lut_export_release */
+} fb_lut_export;
+fb_status fb_atmosphere_export(const fb_atmosphere *a, fb_lut which,
+                               fb_stream stream, fb_lut_export *out);
+/* Closes fd if it is still >= 0 (set it to -1 once passed on) and drops the
+ * library's mapping and handle; the memory lives until every importer has
+ * released it. */
+void fb_lut_export_release(fb_lut_export *e);
+
 /* Drop for Atmosphere (src/precompute.rs:1059-1073). */
 void fb_atmosphere_destroy(fb_atmosphere *a);
 
