@@ -692,48 +692,46 @@ struct DenTexel {
     float osxh, osyh;        // omega_s.x, .y x (NU-1)/2
     float g1, g2;            // 1 + g^2, -2g
     float hnu, hE;           // (NU-1)/2, (IRR_MUS-1)/2
-    float kmax, emax;        // NU-1, IRR_MUS-1
+    float hnu1;              // (NU-1)/2 + 1
     const ProfEntry* erow;   // delta_irradiance at r = bottom
 };
 // Per-(texel, theta) constants.
 struct DenRow {
     float C, omzc;           // cos(theta)*omega_s.z, cos(theta)*omega.z
-    float Ch;                // (C + 1) x (NU-1)/2
+    float Ch1;               // (C + 1) x (NU-1)/2 + 1
     const ProfEntry* pr;     // nu profile of multiple scattering (order n-1 >= 2)
     const PairEntry* pp;     // nu profiles of single Rayleigh and Mie (order 1)
     const float4* dir;       // c_den_dir row: (cos phi sin t, sin phi sin t, cos t, dw)
     float3 gv;               // T_g * albedo / pi (ground rows)
-    float Ag, Bg, Cg;        // ground normal . omega_s = cx*Ag + sy*Bg + Cg, each
-                             // x (IRR_MUS-1)/2, Cg + 1 x (IRR_MUS-1)/2
+    float Ag, Bg, Cg1;       // entry coordinate of the ground normal . omega_s:
+                             // cx*Ag + sy*Bg + Cg1 = (n.omega_s + 1) x
+                             // (IRR_MUS-1)/2 + 1
 };
 
-// Linear filtering at texel coordinate t >= 0 after a clamp at 0.  For t
-// in [0, kmax] this is exactly the CLAMP_TO_EDGE result of prof_lerp (at
-// t = kmax the delta has weight 0).  Coordinates come from unit vectors, so
-// rounding puts t at most a few ulp outside [0, kmax]: below 0 it reads the
-// first value; just above kmax, int(t) is still kmax (the last entry) and
-// the weight of its delta a few ulp.  With t >= 0, truncation is floor, so
-// the slot and weight take one conversion and one fract.  (A max: v_med3
-// issues at half the rate of the other VALU operations.)
-__device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof, float t,
-                                              float kmax) {
-    (void)kmax;
-    t = fmaxf(t, 0.0f);
-    const int k = int(t);
-    const float f = __builtin_amdgcn_fractf(t);
-    const ProfEntry* e = prof + (k + 1);
+// Linear filtering at texel coordinate t, given as e = t + 1: entry k + 1
+// holds slot k, so floor(e) is the entry and fract(e) the weight.  For t in
+// [0, kmax] this is the CLAMP_TO_EDGE result of prof_lerp (at t = kmax the
+// delta has weight 0).  Coordinates come from unit vectors, so rounding puts
+// t at most a few ulp outside [0, kmax]: for t in (-1, 0), e is in (0, 1)
+// and reads entry 0, the clamped left edge (slot 0's value, delta 0); just
+// above kmax, int(e) is still the last entry and the weight of its delta a
+// few ulp.  With e > 0, truncation is floor: the entry and weight take one
+// conversion and one fract, and no clamp.  The +1 is folded into the
+// per-texel and per-row terms the callers add anyway.
+__device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof, float e1) {
+    const int k = int(e1);
+    const float f = __builtin_amdgcn_fractf(e1);
+    const ProfEntry* e = prof + k;
     float4 a = e->a;
     float2 b = e->b;
     return make_float3(fmaf(a.w, f, a.x), fmaf(b.x, f, a.y), fmaf(b.y, f, a.z));
 }
 // The same for a Rayleigh / Mie pair of profiles.
-__device__ __forceinline__ void pair_lerp_i(const PairEntry* __restrict__ prof, float t,
-                                            float kmax, float3& R, float3& M) {
-    (void)kmax;
-    t = fmaxf(t, 0.0f);
-    const int k = int(t);
-    const float f = __builtin_amdgcn_fractf(t);
-    const PairEntry* e = prof + (k + 1);
+__device__ __forceinline__ void pair_lerp_i(const PairEntry* __restrict__ prof, float e1,
+                                            float3& R, float3& M) {
+    const int k = int(e1);
+    const float f = __builtin_amdgcn_fractf(e1);
+    const PairEntry* e = prof + k;
     const float4 a = e->a, b = e->b, c = e->c;
     R = make_float3(fmaf(a.w, f, a.x), fmaf(b.x, f, a.y), fmaf(b.y, f, a.z));
     M = make_float3(fmaf(c.y, f, b.z), fmaf(c.z, f, b.w), fmaf(c.w, f, c.x));
@@ -759,11 +757,12 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
         if (SINGLE) {
             const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
             const float nua = base + sb, nub = base - sb;
-            const float ta = fmaf(nua, dt.hnu, dt.hnu), tb = fmaf(nub, dt.hnu, dt.hnu);
+            // entry coordinates (nu + 1) (NU-1)/2 + 1
+            const float ta = fmaf(nua, dt.hnu, dt.hnu1), tb = fmaf(nub, dt.hnu, dt.hnu1);
             // Profiles are pre-scaled by kR (rayleigh) and kM (mie).
             float3 Ma, Mb;
-            pair_lerp_i(row.pp, ta, dt.kmax, La, Ma);
-            pair_lerp_i(row.pp, tb, dt.kmax, Lb, Mb);
+            pair_lerp_i(row.pp, ta, La, Ma);
+            pair_lerp_i(row.pp, tb, Lb, Mb);
             const float qa = fmaf(nua, nua, 1.0f), qb = fmaf(nub, nub, 1.0f);
             const float wa = rsqrt_cubed(fmaf(dt.g2, nua, dt.g1));
             const float wb = rsqrt_cubed(fmaf(dt.g2, nub, dt.g1));
@@ -772,16 +771,16 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
             Lb = make_float3(qb * fmaf(Mb.x, wb, Lb.x), qb * fmaf(Mb.y, wb, Lb.y),
                              qb * fmaf(Mb.z, wb, Lb.z));
         } else {
-            // texel coordinate (nu + 1) (NU-1)/2 with the scale folded into
-            // the per-texel and per-row terms
-            const float base = fmaf(cx, dt.osxh, row.Ch), sb = sy * dt.osyh;
-            La = prof_lerp_i(row.pr, base + sb, dt.kmax);
-            Lb = prof_lerp_i(row.pr, base - sb, dt.kmax);
+            // entry coordinate (nu + 1) (NU-1)/2 + 1 with the scale and
+            // offset folded into the per-texel and per-row terms
+            const float base = fmaf(cx, dt.osxh, row.Ch1), sb = sy * dt.osyh;
+            La = prof_lerp_i(row.pr, base + sb);
+            Lb = prof_lerp_i(row.pr, base - sb);
         }
         if (GROUND) {
-            const float gb = fmaf(cx, row.Ag, row.Cg), gs = sy * row.Bg;
-            const float3 Ea = prof_lerp_i(dt.erow, gb + gs, dt.emax);
-            const float3 Eb = prof_lerp_i(dt.erow, gb - gs, dt.emax);
+            const float gb = fmaf(cx, row.Ag, row.Cg1), gs = sy * row.Bg;
+            const float3 Ea = prof_lerp_i(dt.erow, gb + gs);
+            const float3 Eb = prof_lerp_i(dt.erow, gb - gs);
             La = make_float3(fmaf(row.gv.x, Ea.x, La.x), fmaf(row.gv.y, Ea.y, La.y),
                              fmaf(row.gv.z, Ea.z, La.z));
             Lb = make_float3(fmaf(row.gv.x, Eb.x, Lb.x), fmaf(row.gv.y, Eb.y, Lb.y),
@@ -990,9 +989,8 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     dt.g1 = 1.0f + g * g;
     dt.g2 = -2.0f * g;
     dt.hnu = 0.5f * float(NU - 1);
+    dt.hnu1 = dt.hnu + 1.0f;
     dt.hE = 0.5f * float(NE - 1);
-    dt.kmax = float(NU - 1);
-    dt.emax = float(NE - 1);
     dt.erow = s_erow;
 
     float3 acc = make_float3(0.0f, 0.0f, 0.0f);
@@ -1002,7 +1000,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         const float ct = d0.z, dom = d0.w;
         DenRow row;
         row.C = ct * dt.osz;
-        row.Ch = fmaf(row.C, dt.hnu, dt.hnu);
+        row.Ch1 = fmaf(row.C, dt.hnu, dt.hnu1);
         row.omzc = ct * dt.omz;
         row.pr = s_prof + l * nent;
         row.pp = s_pair + l * nent;
@@ -1014,7 +1012,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
             row.gv = make_float3(gv.x, gv.y, gv.z);
             row.Ag = dgi * dt.osx * dt.hE;
             row.Bg = dgi * dt.osy * dt.hE;
-            row.Cg = fmaf((rf + ct * gv.w) * inv * dt.osz, dt.hE, dt.hE);
+            row.Cg1 = fmaf((rf + ct * gv.w) * inv * dt.osz, dt.hE, dt.hE + 1.0f);
             density_row<SINGLE, true>(dt, row, aR, aM);
         } else {
             density_row<SINGLE, false>(dt, row, aR, aM);
