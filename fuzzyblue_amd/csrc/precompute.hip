@@ -709,15 +709,15 @@ struct DenRow {
 };
 
 // Linear filtering at texel coordinate t, given as e = t + 1: entry k + 1
-// holds slot k, so floor(e) is the entry and fract(e) the weight.  For t in
-// [0, kmax] this is the CLAMP_TO_EDGE result of prof_lerp (at t = kmax the
-// delta has weight 0).  Coordinates come from unit vectors, so rounding puts
-// t at most a few ulp outside [0, kmax]: for t in (-1, 0), e is in (0, 1)
-// and reads entry 0, the clamped left edge (slot 0's value, delta 0); just
-// above kmax, int(e) is still the last entry and the weight of its delta a
-// few ulp.  With e > 0, truncation is floor: the entry and weight take one
-// conversion and one fract, and no clamp.  The +1 is folded into the
-// per-texel and per-row terms the callers add anyway.
+// holds slot k, so floor(e) is the entry and fract(e) the weight, as
+// prof_lerp computes for t >= -1.  Coordinates come from unit vectors, so
+// rounding puts t at most a few ulp outside [0, kmax]: for t in (-1, 0), e
+// is in (0, 1) and lerps slots -1 and 0, as the reference's two fetches do
+// for a nu coordinate just below its range; just above kmax, int(e) is still
+// the last entry and the weight of its delta a few ulp.  With e > 0,
+// truncation is floor: the entry and weight take one conversion and one
+// fract, and no clamp.  The +1 is folded into the per-texel and per-row
+// terms the callers add anyway.
 __device__ __forceinline__ float3 prof_lerp_i(const ProfEntry* __restrict__ prof, float e1) {
     const int k = int(e1);
     const float f = __builtin_amdgcn_fractf(e1);
@@ -796,10 +796,17 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
     }
 }
 
+// Dynamic LDS of a density block: the raw profiles (float4 per texture,
+// theta row and nu slot), whose space the irradiance entries reuse once the
+// profile entries are built; the entries (PairEntry at order 2, ProfEntry
+// after); the x taps of the NU + 2 slots.  Kept small: at order 2 it sets
+// how many blocks share a CU.
 inline size_t density_lds_bytes(int nu, int i_mus, bool single) {
     const int ntex = single ? 2 : 1;
-    return sizeof(float4) * (size_t(ntex) * kDenTheta * (nu + 2) + i_mus) +
-           sizeof(ProfEntry) * (size_t(ntex) * kDenTheta * (nu + 1) + i_mus + 1);
+    const size_t raw = sizeof(float4) * size_t(ntex) * kDenTheta * (nu + 2);
+    const size_t erow = sizeof(ProfEntry) * size_t(i_mus + 1);
+    const size_t ent = (single ? sizeof(PairEntry) : sizeof(ProfEntry)) * kDenTheta * (nu + 1);
+    return std::max(raw, erow) + ent + sizeof(Taps) * size_t(nu + 2);
 }
 
 // The density geometry table (dg_row / dg_x) of one LUT set; block b < R
@@ -871,7 +878,6 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     __shared__ float4 s_gv[kDenTheta];    // (T_g*albedo/pi).rgb, distance_to_ground
     __shared__ float s_ginv[kDenTheta];   // 1/|r*zenith + omega_i*d_g|
     __shared__ Taps s_ty[kDenTheta];      // mu taps of each theta row
-    __shared__ Taps s_tx[kMaxNu + 2];     // x taps of each nu slot -1..NU
     __shared__ Taps s_tz;                 // r taps of the tile
     __shared__ int s_ground_mask;
 
@@ -882,10 +888,14 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     const int j = by;  // mu_s index within a nu slice
     const int z = bz;  // r index
     const int tid = threadIdx.x;
+    const int raw_f4 = NTEX * kDenTheta * nslots;
     float4* s_raw = s_dyn;                                   // [NTEX][16][nslots]
-    float4* s_eraw = s_raw + NTEX * kDenTheta * nslots;      // [NE]
-    ProfEntry* s_prof = reinterpret_cast<ProfEntry*>(s_eraw + NE);  // [NTEX][16][nent]
-    ProfEntry* s_erow = s_prof + NTEX * kDenTheta * nent;    // [NE+1]
+    ProfEntry* s_erow = reinterpret_cast<ProfEntry*>(s_dyn);  // [NE+1], after step 2
+    float4* s_ent = s_dyn + max(raw_f4, 2 * (NE + 1));
+    ProfEntry* s_prof = reinterpret_cast<ProfEntry*>(s_ent);  // [16][nent] (order >= 3)
+    PairEntry* s_pair = reinterpret_cast<PairEntry*>(s_ent);  // [16][nent] (order 2)
+    Taps* s_tx = reinterpret_cast<Taps*>(SINGLE ? static_cast<void*>(s_pair + kDenTheta * nent)
+                                                : static_cast<void*>(s_prof + kDenTheta * nent));
 
     // 0. Tile geometry from the density geometry table: the mu taps and
     // ground terms of each theta row (they depend on r only), the r taps,
@@ -907,12 +917,6 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
             s_tz = Taps{__float_as_int(c.x), __float_as_int(c.y), c.z};
         }
     }
-    // delta_irradiance at r = bottom (irradiance.h:20-30 with x_r = 0).
-    {
-        Taps ty = linear_taps<G>(tex_from_unit<G>(G(0), P.i_r), P.i_r);
-        for (int x = tid; x < NE; x += blockDim.x)
-            s_eraw[x] = lerp4(ldg4(dE + ty.i0 * NE + x), ldg4(dE + ty.i1 * NE + x), ty.w);
-    }
     __syncthreads();
     // 1. Raw profiles: 16 theta rows x (NU+2) nu slots (-1..NU) per texture.
     {
@@ -932,9 +936,6 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     const float kRs = SINGLE ? 3.0f / (16.0f * kPi) : 1.0f;
     const float kMs =
         SINGLE ? 3.0f / (8.0f * kPi) * (1.0f - P.g * P.g) / (2.0f + P.g * P.g) : 1.0f;
-    // (Order 2: PairEntry [16][nent] in the same space, 48 of its 64 bytes
-    // per (row, entry).)
-    PairEntry* s_pair = reinterpret_cast<PairEntry*>(s_prof);
     for (int e = tid; e < kDenTheta * nent; e += blockDim.x) {
         int row = e / nent, k = e % nent;  // entry k holds slot k-1
         auto scaled = [&](int rr, int kk, float sc) {
@@ -948,9 +949,16 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         else
             s_prof[e] = make_entry(scaled(row, k, kRs), scaled(row, k + 1, kRs));
     }
-    for (int e = tid; e <= NE; e += blockDim.x) {
-        int i0 = min(max(e - 1, 0), NE - 1), i1 = min(e, NE - 1);
-        s_erow[e] = make_entry(s_eraw[i0], s_eraw[i1]);
+    __syncthreads();
+    // 3. delta_irradiance at r = bottom (irradiance.h:20-30 with x_r = 0)
+    // as (value, delta) entries, in the raw profiles' space.
+    {
+        const Taps ty = linear_taps<G>(tex_from_unit<G>(G(0), P.i_r), P.i_r);
+        auto er = [&](int x) {
+            return lerp4(ldg4(dE + ty.i0 * NE + x), ldg4(dE + ty.i1 * NE + x), ty.w);
+        };
+        for (int e = tid; e <= NE; e += blockDim.x)
+            s_erow[e] = make_entry(er(min(max(e - 1, 0), NE - 1)), er(min(e, NE - 1)));
     }
     __syncthreads();
 
