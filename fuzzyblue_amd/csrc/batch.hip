@@ -10,10 +10,13 @@
 // order loop runs to the largest order in the batch; an atmosphere of order
 // k contributes no tiles to the launches of orders above k.  The whole
 // schedule is captured once as a hipGraph and replayed.
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
 #include <memory>
+#include <numeric>
+#include <utility>
 #include <vector>
 
 #include "runtime.hpp"
@@ -48,6 +51,9 @@ struct BatchPlan {
     fbk::DevParams* d_params = nullptr;
     fbk::LutPtrs* d_luts = nullptr;
     std::vector<BatchLaunch> launches;
+    // Per member: its multiple-scattering rows in cache-friendly order
+    // (ms_row_order), computed for the first launch of that pass.
+    std::vector<std::vector<uint32_t>> ms_rows;
     // Percentage of the device's resident-block capacity each launch may
     // hold (100: one block per tile).  Below 100 every launch is persistent,
     // leaving the remaining slots on every CU to other streams.
@@ -90,6 +96,55 @@ std::vector<int> tile_order(const std::vector<fb_parameters>& ps) {
 unsigned ms_tpt(const fbk::DevParams& P, unsigned threads) {
     const unsigned W = unsigned(P.s_nu * P.s_mus);
     return W <= threads ? 1u : (W <= 2 * threads ? 2u : 4u);
+}
+
+// The rows (mu index y, r index z) of one atmosphere's multiple-scattering
+// pass in an order that keeps the density rows they read in L2.  The row's
+// ray starts at radius r in direction mu; its 51 samples read the density
+// LUT rows of the (r, mu) cells the ray crosses (multiple_scattering.comp:
+// 30-54).  Rays with the same impact parameter p = r sqrt(1 - mu^2) lie on
+// one straight line and cross the same cells, so rows sorted by (p, r) put
+// rays that share cells next to each other: consecutive tiles run together
+// on one XCD (contiguous eighths) or on neighbouring XCDs at the same time.
+// r and mu are the texel centres' (scattering.h:96-131, in double; only
+// the order depends on them, never a result).
+std::vector<uint32_t> ms_row_order(const fbk::DevParams& P) {
+    const int MU = P.s_mu, R = P.s_r;
+    const double top = P.top, bottom = P.bottom;
+    const double H = sqrt(top * top - bottom * bottom);
+    auto unit = [](double u, int n) { return (u - 0.5 / n) / (1.0 - 1.0 / n); };
+    // (p, r) packed into one 64-bit key: both are non-negative, so the bit
+    // patterns of their binary32 values sort like the values.
+    std::vector<uint64_t> key(size_t(MU) * R);
+    for (int z = 0; z < R; ++z) {
+        const double rho = H * unit((z + 0.5) / R, R);
+        const double r = sqrt(rho * rho + bottom * bottom);
+        for (int y = 0; y < MU; ++y) {
+            const double uz = (y + 0.5) / MU;
+            double mu;
+            if (uz < 0.5) {
+                const double d_min = r - bottom, d_max = rho;
+                const double d = d_min + (d_max - d_min) * unit(1.0 - 2.0 * uz, MU / 2);
+                mu = d == 0.0 ? -1.0 : -(rho * rho + d * d) / (2.0 * r * d);
+            } else {
+                const double d_min = top - r, d_max = rho + H;
+                const double d = d_min + (d_max - d_min) * unit(2.0 * uz - 1.0, MU / 2);
+                mu = d == 0.0 ? 1.0 : (H * H - rho * rho - d * d) / (2.0 * r * d);
+            }
+            mu = std::min(1.0, std::max(-1.0, mu));
+            const float pk = float(r * sqrt(1.0 - mu * mu)), rk = float(r);
+            uint32_t pb, rb;
+            memcpy(&pb, &pk, 4);
+            memcpy(&rb, &rk, 4);
+            key[size_t(z) * MU + y] = (uint64_t(pb) << 32) | rb;
+        }
+    }
+    std::vector<uint32_t> order(key.size());
+    std::iota(order.begin(), order.end(), 0u);
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return key[a] < key[b] || (key[a] == key[b] && a < b);
+    });
+    return order;
 }
 
 // XCD-contiguous order of a tile list: hardware block i runs on XCD i % 8,
@@ -169,7 +224,14 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
                           pass);
                 return FB_EINVAL;
             }
-            for (unsigned b = 0; b < sh.blocks; ++b) tiles.push_back((uint32_t(i) << 24) | b);
+            if (pass == fbk::kPassMultiple) {
+                if (plan.ms_rows.size() < dps.size()) plan.ms_rows.resize(dps.size());
+                if (plan.ms_rows[i].empty()) plan.ms_rows[i] = ms_row_order(dps[i]);
+                // block index = z * MU + y
+                for (uint32_t b : plan.ms_rows[i]) tiles.push_back((uint32_t(i) << 24) | b);
+            } else {
+                for (unsigned b = 0; b < sh.blocks; ++b) tiles.push_back((uint32_t(i) << 24) | b);
+            }
             l.threads = std::max(l.threads, sh.threads);
             l.lds = std::max(l.lds, sh.lds);
         }
@@ -277,6 +339,8 @@ fb_status build_group(fb_builder* b, hipStream_t s, const std::vector<fb_paramet
         for (int pass : {fbk::kPassDensity, fbk::kPassIndirect, fbk::kPassMultiple})
             if ((st = add_launch(*plan, ps, dps, tus, order_idx, pass, order)) != FB_OK)
                 return fail(st);
+    plan->ms_rows.clear();
+    plan->ms_rows.shrink_to_fit();
     st = upload_plan(b, *plan, dps, luts, s);
     uploaded = plan->block != nullptr;
     if (st != FB_OK) return fail(st);
