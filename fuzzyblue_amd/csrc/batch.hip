@@ -15,7 +15,6 @@
 
 #include <algorithm>
 #include <memory>
-#include <numeric>
 #include <utility>
 #include <vector>
 
@@ -103,47 +102,45 @@ unsigned ms_tpt(const fbk::DevParams& P, unsigned threads) {
 // ray starts at radius r in direction mu; its 51 samples read the density
 // LUT rows of the (r, mu) cells the ray crosses (multiple_scattering.comp:
 // 30-54).  Rays with the same impact parameter p = r sqrt(1 - mu^2) lie on
-// one straight line and cross the same cells, so rows sorted by (p, r) put
-// rays that share cells next to each other: consecutive tiles run together
-// on one XCD (contiguous eighths) or on neighbouring XCDs at the same time.
-// r and mu are the texel centres' (scattering.h:96-131, in double; only
-// the order depends on them, never a result).
+// one straight line and cross the same cells, so rows ordered by p put rays
+// that share cells next to each other: consecutive tiles run together on
+// one XCD (contiguous eighths) or on neighbouring XCDs at the same time.
+// The order only steers caching, never a result, so it is approximate and
+// cheap (it is built on the serving loop's thread): r and mu of the texel
+// centres (scattering.h:96-131) in binary32, p in 4096 buckets (a counting
+// sort), rows of one bucket by increasing r.
 std::vector<uint32_t> ms_row_order(const fbk::DevParams& P) {
+    constexpr int kBuckets = 4096;
     const int MU = P.s_mu, R = P.s_r;
-    const double top = P.top, bottom = P.bottom;
-    const double H = sqrt(top * top - bottom * bottom);
-    auto unit = [](double u, int n) { return (u - 0.5 / n) / (1.0 - 1.0 / n); };
-    // (p, r) packed into one 64-bit key: both are non-negative, so the bit
-    // patterns of their binary32 values sort like the values.
-    std::vector<uint64_t> key(size_t(MU) * R);
+    const float top = P.top, bottom = P.bottom;
+    const float H = sqrtf(top * top - bottom * bottom);
+    auto unit = [](float u, int n) { return (u - 0.5f / n) / (1.0f - 1.0f / n); };
+    std::vector<uint16_t> bucket(size_t(MU) * R);
+    std::vector<uint32_t> start(kBuckets + 1, 0u);
     for (int z = 0; z < R; ++z) {
-        const double rho = H * unit((z + 0.5) / R, R);
-        const double r = sqrt(rho * rho + bottom * bottom);
+        const float rho = H * unit((z + 0.5f) / R, R);
+        const float r = sqrtf(rho * rho + bottom * bottom);
         for (int y = 0; y < MU; ++y) {
-            const double uz = (y + 0.5) / MU;
-            double mu;
-            if (uz < 0.5) {
-                const double d_min = r - bottom, d_max = rho;
-                const double d = d_min + (d_max - d_min) * unit(1.0 - 2.0 * uz, MU / 2);
-                mu = d == 0.0 ? -1.0 : -(rho * rho + d * d) / (2.0 * r * d);
+            const float uz = (y + 0.5f) / MU;
+            float mu;
+            if (uz < 0.5f) {
+                const float d_min = r - bottom, d_max = rho;
+                const float d = d_min + (d_max - d_min) * unit(1.0f - 2.0f * uz, MU / 2);
+                mu = d == 0.0f ? -1.0f : -(rho * rho + d * d) / (2.0f * r * d);
             } else {
-                const double d_min = top - r, d_max = rho + H;
-                const double d = d_min + (d_max - d_min) * unit(2.0 * uz - 1.0, MU / 2);
-                mu = d == 0.0 ? 1.0 : (H * H - rho * rho - d * d) / (2.0 * r * d);
+                const float d_min = top - r, d_max = rho + H;
+                const float d = d_min + (d_max - d_min) * unit(2.0f * uz - 1.0f, MU / 2);
+                mu = d == 0.0f ? 1.0f : (H * H - rho * rho - d * d) / (2.0f * r * d);
             }
-            mu = std::min(1.0, std::max(-1.0, mu));
-            const float pk = float(r * sqrt(1.0 - mu * mu)), rk = float(r);
-            uint32_t pb, rb;
-            memcpy(&pb, &pk, 4);
-            memcpy(&rb, &rk, 4);
-            key[size_t(z) * MU + y] = (uint64_t(pb) << 32) | rb;
+            const float p = r * sqrtf(std::max(0.0f, 1.0f - mu * mu));
+            const int k = std::min(kBuckets - 1, std::max(0, int(p / top * kBuckets)));
+            bucket[size_t(z) * MU + y] = uint16_t(k);
+            ++start[k + 1];
         }
     }
-    std::vector<uint32_t> order(key.size());
-    std::iota(order.begin(), order.end(), 0u);
-    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return key[a] < key[b] || (key[a] == key[b] && a < b);
-    });
+    for (int k = 0; k < kBuckets; ++k) start[k + 1] += start[k];
+    std::vector<uint32_t> order(bucket.size());
+    for (uint32_t i = 0; i < uint32_t(bucket.size()); ++i) order[start[bucket[i]]++] = i;
     return order;
 }
 

@@ -485,6 +485,42 @@ fb_status fb_server_submit_precompute(fb_server* s, const fb_parameters* p, uint
     return FB_OK;
 }
 
+fb_status fb_server_submit_precomputes(fb_server* s, const fb_parameters* ps, size_t n,
+                                       uint64_t* request_ids, uint64_t* lut_ids) {
+    if (!s || (!ps && n)) {
+        set_error("NULL argument");
+        return FB_EINVAL;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        fb_status st = fb_parameters_validate(&ps[i]);
+        if (st != FB_OK) return st;  // nothing is enqueued
+    }
+    std::vector<Req*> rs(n);
+    const double t = now_s();
+    for (size_t i = 0; i < n; ++i) {
+        rs[i] = new Req();
+        rs[i]->kind = FB_REQUEST_PRECOMPUTE;
+        rs[i]->params = ps[i];
+        rs[i]->t_submit = t;
+    }
+    {
+        // One critical section: the worker sees all n together and can admit
+        // them as one batch.
+        std::lock_guard<std::mutex> g(s->mu);
+        for (size_t i = 0; i < n; ++i) {
+            Req* r = rs[i];
+            r->id = s->next_id++;
+            r->lut_id = s->next_lut++;
+            s->waiting.push_back(r);
+            s->stats.submitted++;
+            if (request_ids) request_ids[i] = r->id;
+            if (lut_ids) lut_ids[i] = r->lut_id;
+        }
+    }
+    s->cv_work.notify_all();
+    return FB_OK;
+}
+
 fb_status fb_server_submit_render(fb_server* s, uint64_t lut_id, float altitude_km,
                                   float sun_zenith_rad, uint32_t width, uint32_t height,
                                   uint64_t seed, uint64_t* request_id) {

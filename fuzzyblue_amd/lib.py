@@ -114,6 +114,8 @@ def load():
         "fb_server_destroy": (None, [V]),
         "fb_server_submit_precompute": (
             S, [V, P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "fb_server_submit_precomputes": (S, [V, P, C.c_size_t, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64)]),
         "fb_server_submit_render": (
             S,
             [V, C.c_uint64, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint64,

@@ -36,6 +36,15 @@ class Server:
                                                  C.byref(lid)))
         return rid.value, lid.value
 
+    def submit_precomputes(self, ps):
+        """Submit many precomputes in one call (fb_server_submit_precomputes):
+        [(request_id, lut_id)]."""
+        n = len(ps)
+        arr = (abi.fb_parameters * n)(*[p.to_c() for p in ps])
+        rids, lids = (C.c_uint64 * n)(), (C.c_uint64 * n)()
+        check(load().fb_server_submit_precomputes(self.handle, arr, n, rids, lids))
+        return list(zip(rids, lids))
+
     def submit_render(self, lut_id: int, altitude_km: float, sun_zenith: float, width: int,
                       height: int, seed: int) -> int:
         rid = C.c_uint64()

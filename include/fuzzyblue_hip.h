@@ -537,6 +537,15 @@ void fb_server_destroy(fb_server *s);
 /* Submit a precompute; *lut_id receives the id the LUT set will have. */
 fb_status fb_server_submit_precompute(fb_server *s, const fb_parameters *p,
                                       uint64_t *request_id, uint64_t *lut_id);
+/* Submit n precomputes at once (no counterpart in the reference, which
+ * has no request loop): the loop sees them together and admits as many as
+ * its in-flight cap and memory budget allow as one batch, where one call
+ * per request lets it start on the first ones alone.  All parameter sets
+ * are validated first; on an invalid one nothing is enqueued.  Ids are
+ * written to request_ids[i] / lut_ids[i] (either may be NULL). */
+fb_status fb_server_submit_precomputes(fb_server *s, const fb_parameters *ps,
+                                       size_t n, uint64_t *request_ids,
+                                       uint64_t *lut_ids);
 /* Submit a sky render of width x height synthetic rays (as
  * fb_render_sky_frames, one frame) against resident LUT set lut_id. */
 fb_status fb_server_submit_render(fb_server *s, uint64_t lut_id,

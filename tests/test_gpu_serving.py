@@ -164,3 +164,36 @@ def test_queued_renders_pin_their_set(builder):
     assert st.evictions >= 1
     assert st.hbm_peak <= 3 * res + scratch + 3 * out
     s.close()
+
+
+def test_bulk_submission(builder):
+    """fb_server_submit_precomputes: ids in submission order, every request
+    completes, and each LUT set equals the set a plain build of the same
+    parameters produces (a batch member is bit-identical to a plain build);
+    an invalid parameter set in the list enqueues nothing."""
+    rng = np.random.default_rng(11)
+    ps = [random_atmosphere(rng, True, resolution="half") for _ in range(5)]
+    ps.append(Parameters().replace(**DUMP_SIZES, order=3, flags=abi.FB_FLAG_PRECISE))
+    s = Server(builder, 8 * GB, keep_outputs=True, n_streams=2)
+    ids = s.submit_precomputes(ps)
+    assert len(ids) == len(ps)
+    assert [r for r, _ in ids] == sorted(r for r, _ in ids)
+    s.drain()
+    infos = s.poll()
+    assert len(infos) == len(ps) and all(i.status == abi.FB_OK for i in infos)
+    for p, (_, lut) in zip(ps[-2:], ids[-2:]):
+        pa = Atmosphere.build(builder, None, p)
+        pa.wait()
+        a = pa.assert_ready()
+        got = fetch(s, lut, p)
+        for k, v in got.items():
+            assert np.array_equal(v, a.download(
+                {abi.FB_LUT_TRANSMITTANCE: "transmittance", abi.FB_LUT_IRRADIANCE: "irradiance",
+                 abi.FB_LUT_SCATTERING: "scattering"}[k])), k
+        a.close()
+    bad = ps[0].replace(scattering_nu_size=1)
+    before = s.stats().submitted
+    with pytest.raises(Exception):
+        s.submit_precomputes([ps[1], bad])
+    assert s.stats().submitted == before
+    s.close()
