@@ -95,6 +95,47 @@ def test_passes_fp32_vs_restatement(builder, p):
         pa.close()
 
 
+def test_passes_fp32_default_size_sampled(builder):
+    """FP32 mode at the default LUT sizes (256x64, 64x16, 32x128x32x8), pass
+    by pass on the GPU's own inputs against the FP32 restatement: the 2D
+    passes in full, the 3D passes (single scattering, density of orders 2
+    and 3, multiple scattering) on 4,096 seeded texels each."""
+    p = Parameters()
+    assert p.flags == 0
+    o = Oracle("f32")
+    rng = np.random.default_rng(2024)
+    sw, sh, sd = p.scattering_extent()
+    idx = np.sort(rng.choice(sw * sh * sd, 4096, replace=False))
+    at = lambda a: a.reshape(-1, 4)[idx]  # noqa: E731
+    print(f"\n[parity fp32 default] {len(idx)} texels of {sw}x{sh}x{sd}", flush=True)
+    pa1 = run(builder, p.replace(order=1))
+    out1, scr1 = outputs(pa1), scratch(pa1)
+    T = out1["transmittance"]
+    assert rel_err(T, o.pass_transmittance(p)).max() < 1e-5
+    dE_dir = scr1["delta_irradiance"]
+    assert rel_err(dE_dir, o.pass_direct_irradiance(p, T)).max() < 1e-4
+    ray, mie = o.single_texels(p, T, idx)
+    assert rel_err(at(scr1["delta_rayleigh"]), ray).max() < 1e-4
+    assert rel_err(at(scr1["delta_mie"]), mie).max() < 1e-4
+    dR_g, dM_g = scr1["delta_rayleigh"], scr1["delta_mie"]
+    zeros = np.zeros_like(dR_g)
+    pa2 = run(builder, p.replace(order=2))
+    scr2 = scratch(pa2)
+    J2 = o.density_texels(p, 2, T, dR_g, dM_g, zeros, dE_dir, idx)
+    assert rel_err(at(scr2["scattering_density"]), J2).max() < 1e-4
+    dE1, _ = o.pass_indirect(p, 1, dR_g, dM_g, zeros, np.zeros_like(dE_dir))
+    assert rel_err(scr2["delta_irradiance"], dE1).max() < 1e-4
+    dMS2, _ = o.multiple_texels(p, T, scr2["scattering_density"], idx)
+    assert rel_err(at(scr2["delta_multiple_scattering"]), dMS2).max() < 1e-4
+    pa3 = run(builder, p.replace(order=3))
+    scr3 = scratch(pa3)
+    J3 = o.density_texels(p, 3, T, dR_g, dM_g, scr2["delta_multiple_scattering"],
+                          scr2["delta_irradiance"], idx)
+    assert rel_err(at(scr3["scattering_density"]), J3).max() < 1e-4
+    for pa in (pa1, pa2, pa3):
+        pa.close()
+
+
 def test_full_precise_vs_fp64(builder):
     """Precision mode end to end against the FP64 oracle at the smoke sizes."""
     p = SMOKE.replace(flags=abi.FB_FLAG_PRECISE)
