@@ -88,13 +88,19 @@ struct Tuning {
 // pass of the order-n iteration (n - 1 bounces), multiple n.
 unsigned default_threads(const DevParams& P, int pass, int order);
 // Single scattering stages the r-interpolated transmittance row of each
-// sample in LDS (two rows, double buffered) when the LUT is at most
-// kSsMaxStagedMu wide (64 KiB), with at most kSsTexelsPerThread texels per
-// thread; wider transmittance LUTs take the kernel that reads its taps
-// directly.
+// sample in LDS (two rows, double buffered), with at most
+// kSsTexelsPerThread texels per thread, when the row of texels is wide
+// enough to pay for staging a whole transmittance row per sample — at
+// least two texels per three transmittance mu columns (measured: staged
+// 1.2x faster at 256 texels per 256 columns, 3.5x at 1024 per 512;
+// unstaged 1.5x faster at 64 per 128, 2x at 16 per 256) — and the LUT is
+// at most kSsMaxStagedMu wide (64 KiB).  Otherwise each texel reads its
+// transmittance taps directly.
 constexpr int kSsTexelsPerThread = 4;
 constexpr int kSsMaxStagedMu = 2048;
-inline bool ss_staged(const DevParams& P) { return P.t_mu <= kSsMaxStagedMu; }
+inline bool ss_staged(const DevParams& P) {
+    return P.t_mu <= kSsMaxStagedMu && 2 * P.t_mu <= 3 * P.s_nu * P.s_mus;
+}
 inline size_t ss_staged_lds_bytes(int t_mu) { return 2 * size_t(t_mu) * 16; }
 
 unsigned tuned_threads(const DevParams& P, int pass, int order, const Tuning& tu);
