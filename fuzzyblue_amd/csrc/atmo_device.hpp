@@ -24,7 +24,7 @@ struct DevLayer {
 };
 
 // Mirrors shaders/params.h:26-87 (AtmosphereParameters), field for field,
-// plus the texture sizes.  Passed by value as a kernel argument (248 B) or
+// plus the texture sizes.  Passed by value as a kernel argument (252 B) or
 // read from an HBM array in the batched passes.
 struct DevParams {
     float solar[3];
@@ -40,6 +40,7 @@ struct DevParams {
     float absorption_extinction[3];
     int flags;
     int t_mu, t_r, s_r, s_mu, s_mus, s_nu, i_mus, i_r;
+    int den_split;        // scattering density: threads per texel (1, or kDenTheta)
     DevLayer prof[3][2];  // rayleigh, mie, absorption
 };
 

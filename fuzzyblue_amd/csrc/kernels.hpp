@@ -97,6 +97,11 @@ unsigned default_threads(const DevParams& P, int pass, int order);
 // at most kSsMaxStagedMu wide (64 KiB).  Otherwise each texel reads its
 // transmittance taps directly.
 constexpr int kSsTexelsPerThread = 4;
+// Scattering density with fewer texels than this runs kDenTheta threads per
+// texel, one theta row each, summed by a lane reduction: one thread per
+// texel would leave most of the device idle, each thread walking all 512
+// directions in sequence.
+constexpr size_t kDenSplitTexels = 65536;
 constexpr int kSsMaxStagedMu = 2048;
 inline bool ss_staged(const DevParams& P) {
     return P.t_mu <= kSsMaxStagedMu && 2 * P.t_mu <= 3 * P.s_nu * P.s_mus;

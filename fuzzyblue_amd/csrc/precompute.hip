@@ -962,8 +962,12 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     }
     __syncthreads();
 
-    const int t = bx * blockDim.x + tid;  // texel within the tile
-    if (t >= MU * NU) return;
+    // S consecutive lanes per texel, lane l0 of a texel taking theta rows
+    // l0, l0 + S, ... (S = 1: one thread walks all 16 rows).
+    const int S = P.den_split;
+    const int t = bx * (int(blockDim.x) / S) + tid / S;  // texel within the tile
+    if (t >= MU * NU) return;  // a whole group of S lanes
+    const int l0 = tid % S;
     const int k = t % NU, y = t / NU;
     const unsigned x = unsigned(k * MUS + j);
     TexelCoord<G> c = scattering_texel_coord<G>(P, x, unsigned(y), unsigned(z));
@@ -1002,7 +1006,7 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     dt.erow = s_erow;
 
     float3 acc = make_float3(0.0f, 0.0f, 0.0f);
-    for (int l = 0; l < kDenTheta; ++l) {
+    for (int l = l0; l < kDenTheta; l += S) {
         float3 aR = make_float3(0.0f, 0.0f, 0.0f), aM = make_float3(0.0f, 0.0f, 0.0f);
         const float4 d0 = c_den_dir[l * kDenPhi];
         const float ct = d0.z, dom = d0.w;
@@ -1029,6 +1033,16 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         acc.x = fmaf(cm0 * dom, aM.x, fmaf(cr0 * dom, aR.x, acc.x));
         acc.y = fmaf(cm1 * dom, aM.y, fmaf(cr1 * dom, aR.y, acc.y));
         acc.z = fmaf(cm2 * dom, aM.z, fmaf(cr2 * dom, aR.z, acc.z));
+    }
+    if (S > 1) {
+        // The S rows' sums of one texel, in S consecutive lanes of one
+        // wavefront (S divides 64): a tree reduction.
+        for (int o = S / 2; o > 0; o >>= 1) {
+            acc.x += __shfl_xor(acc.x, o);
+            acc.y += __shfl_xor(acc.y, o);
+            acc.z += __shfl_xor(acc.z, o);
+        }
+        if (l0 != 0) return;
     }
     st_tex(J, (size_t(z) * MU + y) * W + x, make_float4(acc.x, acc.y, acc.z, 0.0f), H);
 }
@@ -1535,7 +1549,8 @@ __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams*
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs l = lut_ptrs(L, atm);
-        const int gx = (P.s_mu * P.s_nu + int(blockDim.x) - 1) / int(blockDim.x);
+        const int per = int(blockDim.x) / P.den_split;  // texels per block
+        const int gx = (P.s_mu * P.s_nu + per - 1) / per;
         scattering_density_block<G, SINGLE, H>(P, local % gx, (local / gx) % P.s_mus,
                                                local / (gx * P.s_mus), l.DG, l.dR, l.dM, l.dMS,
                                                l.dE, l.J);
@@ -1685,7 +1700,7 @@ static hipError_t launch_scattering_density_h(const DevParams& P, int order, con
                                               const Tuning& tu, hipStream_t s) {
     const unsigned threads = tuned_threads(P, kPassDensity, order, tu);
     const unsigned remap = (tu.remap >> kPassDensity) & 1u;
-    dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, threads), P.s_mus, P.s_r);
+    dim3 grid(cdiv(size_t(P.s_mu) * P.s_nu, threads / unsigned(P.den_split)), P.s_mus, P.s_r);
     // The density of order n reads the scattering of order n-1; order 1 is
     // single Rayleigh + Mie (scattering.h:157-179).
     const bool single = (order - 1) == 1;
@@ -1841,7 +1856,8 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
         case kPassGeometry:
             return {unsigned(P.s_r + P.s_mus + P.s_mu * P.s_r), 64, 0};
         case kPassDensity:
-            return {cdiv(size_t(P.s_mu) * P.s_nu, threads) * unsigned(P.s_mus) * unsigned(P.s_r),
+            return {cdiv(size_t(P.s_mu) * P.s_nu, threads / unsigned(P.den_split)) *
+                        unsigned(P.s_mus) * unsigned(P.s_r),
                     threads, density_lds_bytes(P.s_nu, P.i_mus, (order - 1) == 1)};
         case kPassIndirect: {  // one wave per texel
             const unsigned waves = threads / 64;

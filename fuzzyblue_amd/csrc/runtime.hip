@@ -133,6 +133,9 @@ fbk::DevParams make_dev_params(const fb_parameters& p) {
     d.s_nu = int(p.scattering_nu_size);
     d.i_mus = int(p.irradiance_mu_s_size);
     d.i_r = int(p.irradiance_r_size);
+    const size_t texels = size_t(p.scattering_r_size) * p.scattering_mu_size *
+                          p.scattering_mu_s_size * p.scattering_nu_size;
+    d.den_split = texels < fbk::kDenSplitTexels ? fbk::kDenTheta : 1;
     const fb_density_profile* src[3] = {&p.rayleigh_density, &p.mie_density,
                                         &p.absorption_density};
     for (int k = 0; k < 3; ++k)
