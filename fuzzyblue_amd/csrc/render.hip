@@ -150,15 +150,17 @@ __host__ __device__ inline size_t sky_slice_bytes(const DevParams& P) {
 }
 constexpr size_t kSkySliceMaxBytes = 65536;
 
+// Entries first, first + step, ... of the slice at (r, mu_s) into `lds`
+// (the block's LDS, or a frame's slice in global memory).
 template <bool H>
 __device__ void build_sky_slice(const DevParams& P, const float4* __restrict__ T,
                                 const float4* __restrict__ S, float r, float mu_s,
-                                float4* __restrict__ lds) {
+                                float4* __restrict__ lds, int first, int step) {
     const int NU = P.s_nu, MU = P.s_mu, W = NU * P.s_mus, TMU = P.t_mu;
     const Taps tz = linear_taps<float>(scattering_u_r<float>(P, r), P.s_r);
     const float u_mu_s = scattering_u_mu_s<float>(P, mu_s);
     constexpr bool h = H;
-    for (int e = threadIdx.x; e < (NU + 1) * MU; e += blockDim.x) {
+    for (int e = first; e < (NU + 1) * MU; e += step) {
         const int k = e / MU, y = e - k * MU;
         const Taps tx = linear_taps<float>((float(k) + u_mu_s) / float(NU), W);
         const size_t p0 = (size_t(tz.i0) * MU + y) * W;
@@ -170,30 +172,126 @@ __device__ void build_sky_slice(const DevParams& P, const float4* __restrict__ T
     transmittance_uv<float>(P, r, 1.0f, u, v);  // v depends on r only
     const Taps tv = linear_taps<float>(v, P.t_r);
     float4* trow = lds + (NU + 1) * MU;
-    for (int m = threadIdx.x; m < TMU; m += blockDim.x)
+    for (int m = first; m < TMU; m += step)
         trow[m] = lerp4(ldg4(T + tv.i0 * TMU + m), ldg4(T + tv.i1 * TMU + m), tv.w);
 }
 
+// a / b correctly rounded, given y = 1/b correctly rounded and b normal: q =
+// a*y, then Markstein's correction q + (a - b*q)*y with the residual exact
+// in an FMA (P. Markstein, "Computation of elementary functions on the IBM
+// RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990) -- three
+// instructions in place of an IEEE division's ten, the same bits.  The
+// residual is a multiple of about 2^(e_a - 46): for numerators below 2^-80
+// (other than 0) it can be subnormal or not representable, and a wavefront
+// with such a lane takes the division (a wave-uniform branch; the empty asm
+// statement keeps the compiler from evaluating the division unconditionally
+// and selecting).
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+    const float q = a * y;
+    float d = fmaf(fmaf(-q, b, a), y, q);
+    if (__builtin_amdgcn_ballot_w64(a != 0.0f && fabsf(a) < 0x1p-80f)) {
+        asm volatile("");
+        d = a / b;
+    }
+    return d;
+}
+
+// The per-frame terms of sky_radiance_slice: everything that depends on
+// the camera's r and the atmosphere only, computed once per block.
+struct SliceConsts {
+    float r, r_inv, rr, b2, t2;   // r, 1/r, r*r, bottom^2, top^2
+    float t_dmin, t_den, t_inv;   // transmittance x_mu = (d - t_dmin) / t_den
+    float t_u0, t_u1;             // tex_from_unit(x, t_mu) = t_u0 + x * t_u1
+    float g_dmin, g_den, g_inv;   // scattering u_mu, rays to the ground
+    float s_dmin, s_den, s_inv;   // ... and to the top
+    float h2;                     // H^2
+    float m_u0, m_u1;             // tex_from_unit(x, MU/2)
+    float mie[3];                 // (rayleigh.r / mie.r) * (mie / rayleigh)
+    float pm_k, pm_g1, pm_g2;     // phase_mie: k, 1 + g^2, 2g
+};
+__device__ inline SliceConsts slice_consts(const DevParams& P, float r) {
+    SliceConsts c;
+    const float b = P.bottom, t = P.top;
+    c.r = r;
+    c.r_inv = 1.0f / r;
+    c.rr = r * r;
+    c.b2 = b * b;
+    c.t2 = t * t;
+    const float H = sqrtf(t * t - b * b);
+    const float rho = sqrtf(fmaxf(r * r - b * b, 0.0f));
+    // transmittance.h:7-24
+    c.t_dmin = t - r;
+    c.t_den = rho + H - c.t_dmin;
+    c.t_inv = 1.0f / c.t_den;
+    c.t_u0 = 0.5f / float(P.t_mu);
+    c.t_u1 = 1.0f - 1.0f / float(P.t_mu);
+    // scattering.h:23-46 (d_max == d_min for the ground: x = 0)
+    c.g_dmin = r - b;
+    c.g_den = rho - c.g_dmin;
+    c.g_inv = 1.0f / c.g_den;
+    c.s_dmin = t - r;
+    c.s_den = rho + H - c.s_dmin;
+    c.s_inv = 1.0f / c.s_den;
+    c.h2 = H * H;
+    const int hm = P.s_mu / 2;
+    c.m_u0 = 0.5f / float(hm);
+    c.m_u1 = 1.0f - 1.0f / float(hm);
+    // render_sky.h:9-19
+    const float k = P.rayleigh_scattering[0] / P.mie_scattering[0];
+    for (int i = 0; i < 3; ++i) c.mie[i] = k * (P.mie_scattering[i] / P.rayleigh_scattering[i]);
+    // util.h:31-34
+    const float g = P.g;
+    c.pm_k = 3.0f / (8.0f * kPi) * (1.0f - g * g) / (2.0f + g * g);
+    c.pm_g1 = 1.0f + g * g;
+    c.pm_g2 = 2.0f * g;
+    return c;
+}
+
 // sky_radiance for a camera inside the atmosphere (so no move to the top
-// boundary), from the block's slice at the camera's r and mu_s.
-__device__ void sky_radiance_slice(const DevParams& P, const float4* __restrict__ lds, float r,
-                                   float3 camera, float3 view, float3 sun, float3& rad,
-                                   float3& trans) {
+// boundary), from the block's slice at the camera's r and mu_s.  Every
+// lookup coordinate is the reference's expression in the same binary32
+// operations, rounded the same (IEEE square roots; quotients by a per-frame
+// denominator through div_rn, bit for bit the IEEE quotient): the ground
+// test switches between the two halves of the scattering texture, a ray
+// distance is a difference of two ~r-sized terms (down from 0.5 km: d = 0.5
+// from terms near 6360), and near the horizon the LUTs change steeply
+// between texels, so an ulp in a coordinate can move a value by 1e-3.
+// After the lookups the arithmetic is well conditioned: extrapolated Mie's
+// c.w / c.x and phase_mie's x^-1.5 use the hardware's reciprocal and rsq^3,
+// a few ulp from the IEEE forms.
+__device__ void sky_radiance_slice(const DevParams& P, const SliceConsts& C,
+                                   const float4* __restrict__ lds, float3 camera, float3 view,
+                                   float3 sun, float3& rad, float3& trans) {
     const int NU = P.s_nu, MU = P.s_mu;
-    const float mu = dot3(camera, view) / r;
+    const float r = C.r;
+    const float mu = div_rn(dot3(camera, view), r, C.r_inv);
     const float nu = dot3(view, sun);
     const bool ground = ray_intersects_ground<float>(P, r, mu);
     if (ground) {
         trans = make_float3(0.0f, 0.0f, 0.0f);
     } else {
-        float u, v;
-        transmittance_uv<float>(P, r, mu, u, v);
+        // distance_to_top (params.h:105-110), then transmittance.h:7-24's u
+        const float disc = C.rr * (mu * mu - 1.0f) + C.t2;
+        const float d = fmaxf(-r * mu + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+        const float u = C.t_u0 + div_rn(d - C.t_dmin, C.t_den, C.t_inv) * C.t_u1;
         const Taps tu = linear_taps<float>(u, P.t_mu);
         const float4* trow = lds + (NU + 1) * MU;
         const float4 t = lerp4(trow[tu.i0], trow[tu.i1], tu.w);
         trans = make_float3(t.x, t.y, t.z);
     }
-    const Taps ty = linear_taps<float>(scattering_u_mu<float>(P, r, mu, ground), MU);
+    // scattering.h:23-46
+    const float r_mu = r * mu;
+    const float disc = r_mu * r_mu - C.rr + C.b2;
+    float u_mu;
+    if (ground) {
+        const float d = -r_mu - sqrtf(fmaxf(disc, 0.0f));
+        const float x = C.g_den == 0.0f ? 0.0f : div_rn(d - C.g_dmin, C.g_den, C.g_inv);
+        u_mu = 0.5f - 0.5f * (C.m_u0 + x * C.m_u1);
+    } else {
+        const float d = -r_mu + sqrtf(fmaxf(disc + C.h2, 0.0f));
+        u_mu = 0.5f + 0.5f * (C.m_u0 + div_rn(d - C.s_dmin, C.s_den, C.s_inv) * C.m_u1);
+    }
+    const Taps ty = linear_taps<float>(u_mu, MU);
     // nu of unit vectors can round just below -1 (anti-solar rays): clamped
     // so the slice index stays >= 0.  The per-ray lookup's CLAMP_TO_EDGE
     // then gives the same value up to a weight of the rounding's size.
@@ -205,8 +303,15 @@ __device__ void sky_radiance_slice(const DevParams& P, const float4* __restrict_
     const float4 a1 = lerp4(lds[(k0 + 1) * MU + ty.i0], lds[(k0 + 1) * MU + ty.i1], ty.w);
     const float4 c = make_float4(a0.x * (1.0f - l) + a1.x * l, a0.y * (1.0f - l) + a1.y * l,
                                  a0.z * (1.0f - l) + a1.z * l, a0.w * (1.0f - l) + a1.w * l);
-    const float3 mie = extrapolated_single_mie(P, c);
-    const float pr = phase_rayleigh(nu), pm = phase_mie(P.g, nu);
+    // render_sky.h:9-19
+    float3 mie = make_float3(0.0f, 0.0f, 0.0f);
+    if (c.x > 0.0f) {
+        const float q = c.w * __builtin_amdgcn_rcpf(c.x);
+        mie = make_float3(c.x * q * C.mie[0], c.y * q * C.mie[1], c.z * q * C.mie[2]);
+    }
+    const float pr = phase_rayleigh(nu);
+    const float y = __builtin_amdgcn_rsqf(C.pm_g1 - C.pm_g2 * nu);
+    const float pm = C.pm_k * (1.0f + nu * nu) * (y * y * y);
     rad = make_float3(c.x * pr + mie.x * pm, c.y * pr + mie.y * pm, c.z * pr + mie.z * pm);
 }
 
@@ -325,22 +430,84 @@ __host__ __device__ inline void ray_direction(uint64_t seed, uint32_t frame, uin
 // Pixels per thread of the frame kernels (a block shares one slice).
 constexpr int kFramePixelsPerThread = 16;
 
+// The camera of a synthetic frame at altitude alt: on the z axis, at
+// radius r.  A frame takes the slice path when the LUTs' slice fits in LDS
+// (`slice`) and the camera is inside the atmosphere; the host's choice of
+// kernel and the kernels' own checks use this one test.
+__host__ __device__ inline float frame_camera_z(const DevParams& P, float alt) {
+    return P.bottom + alt;
+}
+__host__ __device__ inline bool frame_uses_slice(const DevParams& P, float alt, bool slice) {
+    const float z = frame_camera_z(P, alt);
+    return slice && sqrtf(0.0f * 0.0f + 0.0f * 0.0f + z * z) <= P.top;
+}
+
+// The frame's camera position, sun direction and camera radius.
+struct FrameView {
+    float3 cam, sun;
+    float r;
+};
+__device__ inline FrameView frame_view(const DevParams& P, float alt, float zs) {
+    FrameView v;
+    v.cam = make_float3(0.0f, 0.0f, frame_camera_z(P, alt));
+    v.sun = make_float3(sinf(zs), 0.0f, cosf(zs));
+    v.r = sqrtf(dot3(v.cam, v.cam));
+    return v;
+}
+__host__ __device__ inline size_t sky_slice_float4s(const DevParams& P) {
+    return sky_slice_bytes(P) / sizeof(float4);
+}
+
+// The LUT slice of every frame on the slice path, once per frame, into
+// `out` (frame f at out + f * sky_slice_float4s): blockIdx.y = frame, the
+// gridDim.x blocks of a frame share its entries.  A frame kernel's blocks
+// then copy their frame's slice into LDS with coalesced loads instead of
+// each gathering it from the scattering LUT (two texel rows per entry).
+template <bool H>
+__global__ __launch_bounds__(256) void k_frame_slices(DevParams P, const float4* __restrict__ T,
+                                                      const float4* __restrict__ S,
+                                                      const float* __restrict__ altitudes,
+                                                      const float* __restrict__ zeniths,
+                                                      float4* __restrict__ out) {
+    const uint32_t f = blockIdx.y;
+    const float alt = altitudes[f];
+    if (!frame_uses_slice(P, alt, true)) return;
+    const FrameView v = frame_view(P, alt, zeniths[f]);
+    build_sky_slice<H>(P, T, S, v.r, dot3(v.cam, v.sun) / v.r, out + f * sky_slice_float4s(P),
+                       int(blockIdx.x * blockDim.x + threadIdx.x),
+                       int(gridDim.x * blockDim.x));
+}
+constexpr unsigned kFrameSliceBlocks = 8;  // blocks per frame of k_frame_slices
+
 // Pixels [pix0, pix0 + 256*kFramePixelsPerThread) of one synthetic frame
 // (camera at altitude alt on the z axis, sun at zenith angle zs in the x-z
 // plane).  `slice`: the block's LDS holds room for the frame's LUT slice.
-template <bool H>
+// SLICE: the kernel for frames on the slice path, else the one for the
+// others (per-ray lookups); a block of the other kind returns at once.  Two
+// kernels rather than one branch: the per-ray lookup's registers would
+// halve the slice kernel's occupancy.  `frame_slice`: the frame's slice
+// from k_frame_slices, or null to build it in the block.
+template <bool H, bool SLICE>
 __device__ void render_frame_block(const DevParams& P, const float4* __restrict__ T,
                                    const float4* __restrict__ S, float alt, float zs,
                                    uint32_t frame, uint32_t pixels, uint64_t seed, uint32_t pix0,
-                                   bool slice, float4* __restrict__ rad_out,
+                                   bool slice, const float4* __restrict__ frame_slice,
+                                   float4* __restrict__ rad_out,
                                    float4* __restrict__ trans_out) {
     extern __shared__ float4 s_slice[];
-    const float3 cam = make_float3(0.0f, 0.0f, P.bottom + alt);
-    const float3 sun = make_float3(sinf(zs), 0.0f, cosf(zs));
-    const float r = sqrtf(dot3(cam, cam));
-    const bool use_slice = slice && r <= P.top;  // block-uniform
-    if (use_slice) {
-        build_sky_slice<H>(P, T, S, r, dot3(cam, sun) / r, s_slice);
+    if (frame_uses_slice(P, alt, slice) != SLICE) return;  // block-uniform
+    const FrameView v = frame_view(P, alt, zs);
+    const float3 cam = v.cam, sun = v.sun;
+    const float r = v.r;
+    SliceConsts sc;
+    if constexpr (SLICE) {
+        if (frame_slice) {
+            const int n = int(sky_slice_float4s(P));
+            for (int i = threadIdx.x; i < n; i += blockDim.x) s_slice[i] = frame_slice[i];
+        } else {
+            build_sky_slice<H>(P, T, S, r, dot3(cam, sun) / r, s_slice, threadIdx.x, blockDim.x);
+        }
+        sc = slice_consts(P, r);
         __syncthreads();
     }
     for (int j = 0; j < kFramePixelsPerThread; ++j) {
@@ -350,28 +517,36 @@ __device__ void render_frame_block(const DevParams& P, const float4* __restrict_
         ray_direction(seed, frame, p, d);
         const float3 view = make_float3(d[0], d[1], d[2]);
         float3 rad, tr;
-        if (use_slice)
-            sky_radiance_slice(P, s_slice, r, cam, view, sun, rad, tr);
+        if constexpr (SLICE)
+            sky_radiance_slice(P, sc, s_slice, cam, view, sun, rad, tr);
         else
             sky_radiance<H>(P, T, S, cam, view, sun, rad, tr);
         const size_t o = size_t(frame) * pixels + p;  // frame-major output
-        rad_out[o] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-        if (trans_out) trans_out[o] = make_float4(tr.x, tr.y, tr.z, 1.0f);
+        // Written once and not read back by the kernel: non-temporal stores.
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(v4f{rad.x, rad.y, rad.z, 0.0f},
+                                    reinterpret_cast<v4f*>(rad_out + o));
+        if (trans_out)
+            __builtin_nontemporal_store(v4f{tr.x, tr.y, tr.z, 1.0f},
+                                        reinterpret_cast<v4f*>(trans_out + o));
     }
 }
 
-template <bool H>
+template <bool H, bool SLICE>
 __global__ __launch_bounds__(256) void k_render_frames(DevParams P, const float4* __restrict__ T,
                                                        const float4* __restrict__ S,
                                                        const float* __restrict__ altitudes,
                                                        const float* __restrict__ zeniths,
                                                        uint32_t pixels, uint64_t seed,
-                                                       int slice, float4* __restrict__ rad_out,
+                                                       int slice,
+                                                       const float4* __restrict__ slices,
+                                                       float4* __restrict__ rad_out,
                                                        float4* __restrict__ trans_out) {
     const uint32_t f = blockIdx.y;
-    render_frame_block<H>(P, T, S, altitudes[f], zeniths[f], f, pixels, seed,
-                       blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0, rad_out,
-                       trans_out);
+    render_frame_block<H, SLICE>(P, T, S, altitudes[f], zeniths[f], f, pixels, seed,
+                                 blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0,
+                                 slices ? slices + f * sky_slice_float4s(P) : nullptr, rad_out,
+                                 trans_out);
 }
 
 // render_lighting.h:10-28
@@ -399,14 +574,14 @@ __global__ __launch_bounds__(256) void k_sun_sky_irradiance(
 
 // One synthetic frame with scalar camera altitude / sun zenith (the serving
 // loop's render request; same rays as frame 0 of k_render_frames).
-template <bool H>
+template <bool H, bool SLICE>
 __global__ __launch_bounds__(256) void k_render_frame1(DevParams P, const float4* __restrict__ T,
                                                        const float4* __restrict__ S, float alt,
                                                        float zs, uint32_t pixels, uint64_t seed,
                                                        int slice, float4* __restrict__ rad_out) {
-    render_frame_block<H>(P, T, S, alt, zs, 0u, pixels, seed,
-                       blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0, rad_out,
-                       nullptr);
+    render_frame_block<H, SLICE>(P, T, S, alt, zs, 0u, pixels, seed,
+                                 blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0,
+                                 nullptr, rad_out, nullptr);
 }
 
 // Blocks and dynamic LDS of a frame launch.
@@ -424,12 +599,16 @@ hipError_t launch_render_frame(const DevParams& P, const float4* T, const float4
                                hipStream_t s) {
     if (pixels == 0) return hipSuccess;
     const size_t lds = frame_lds(P);
-    if (P.flags & kFlagF16Storage)
-        k_render_frame1<true><<<frame_blocks(pixels), 256, lds, s>>>(P, T, S, alt, zs, pixels,
-                                                                     seed, lds ? 1 : 0, rad);
+    const int slice = lds ? 1 : 0;
+    const unsigned g = frame_blocks(pixels);
+#define FB_FRAME1(H, SL) \
+    k_render_frame1<H, SL><<<g, 256, SL ? lds : 0, s>>>(P, T, S, alt, zs, pixels, seed, slice, rad)
+    const bool h = P.flags & kFlagF16Storage;
+    if (frame_uses_slice(P, alt, slice != 0))
+        h ? FB_FRAME1(true, true) : FB_FRAME1(false, true);
     else
-        k_render_frame1<false><<<frame_blocks(pixels), 256, lds, s>>>(P, T, S, alt, zs, pixels,
-                                                                      seed, lds ? 1 : 0, rad);
+        h ? FB_FRAME1(true, false) : FB_FRAME1(false, false);
+#undef FB_FRAME1
     return hipGetLastError();
 }
 
@@ -550,14 +729,44 @@ fb_status fb_render_sky_frames(fb_builder* b, fb_stream stream, const fb_atmosph
     const size_t lds = fbk::frame_lds(a->dev);
     auto* r4 = reinterpret_cast<float4*>(radiance);
     auto* t4 = reinterpret_cast<float4*>(transmittance);
-    if (a->dev.flags & fbk::kFlagF16Storage)
-        fbk::k_render_frames<true><<<grid, 256, lds, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
-                                                           sun_zenith_rad, pixels, seed,
-                                                           lds ? 1 : 0, r4, t4);
-    else
-        fbk::k_render_frames<false><<<grid, 256, lds, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
-                                                            sun_zenith_rad, pixels, seed,
-                                                            lds ? 1 : 0, r4, t4);
+    const int slice = lds ? 1 : 0;
+    const bool h = a->dev.flags & fbk::kFlagF16Storage;
+    // The altitudes are on the device: k_frame_slices and the slice kernel
+    // take the frames on the slice path, the per-ray kernel the others (when
+    // no slice fits, all of them); each returns at once from the other's
+    // frames.  The frames' slices live in stream-ordered scratch
+    // (n_frames * at most 64 KB), freed after the launches that read them.
+    float4* slices = nullptr;
+    if (slice) {
+        hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&slices), size_t(n_frames) * lds, s);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("frame slice scratch: %s", hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? FB_ENOMEM : FB_EHIP;
+        }
+        dim3 sg(fbk::kFrameSliceBlocks, n_frames);
+        if (h)
+            fbk::k_frame_slices<true><<<sg, 256, 0, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
+                                                          sun_zenith_rad, slices);
+        else
+            fbk::k_frame_slices<false><<<sg, 256, 0, s>>>(a->dev, a->T.p, a->S.p, altitudes_km,
+                                                           sun_zenith_rad, slices);
+    }
+#define FB_FRAMES(H, SL)                                                                       \
+    fbk::k_render_frames<H, SL><<<grid, 256, SL ? lds : 0, s>>>(a->dev, a->T.p, a->S.p,        \
+                                                               altitudes_km, sun_zenith_rad, \
+                                                               pixels, seed, slice, slices, r4, t4)
+    if (slice) h ? FB_FRAMES(true, true) : FB_FRAMES(false, true);
+    h ? FB_FRAMES(true, false) : FB_FRAMES(false, false);
+#undef FB_FRAMES
+    if (slices) {
+        const hipError_t le = hipGetLastError();
+        const hipError_t fe = hipFreeAsync(slices, s);
+        if (le != hipSuccess || fe != hipSuccess) {
+            set_error("frame launch: %s", hipGetErrorString(le != hipSuccess ? le : fe));
+            return FB_EHIP;
+        }
+    }
     return end_use(a, s);
 }
 
