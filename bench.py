@@ -180,7 +180,7 @@ def max_err_vs_fixture(atm, floor=1e-6):
             "scattering": rel(S[:, :3], g["S"][:, :3])}
 
 
-def measure_render(b, stream, atm, frames=64, w=1920, h=1080, reps=3):
+def measure_render(b, stream, atm, frames=64, w=1920, h=1080, reps=10, warm=3):
     rng = np.random.default_rng(2)
     alt = rng.uniform(0.0, 10.0, frames).astype(np.float32)
     zen = np.radians(rng.uniform(0.0, 108.0, frames)).astype(np.float32)
@@ -190,7 +190,8 @@ def measure_render(b, stream, atm, frames=64, w=1920, h=1080, reps=3):
     rad = DeviceBuffer(frames * px * 16)
     tr = DeviceBuffer(frames * px * 16)
     r = Renderer(b)
-    r.frames(stream, atm, d_alt, d_zen, frames, px, 2, rad, tr)
+    for _ in range(warm):
+        r.frames(stream, atm, d_alt, d_zen, frames, px, 2, rad, tr)
     synchronize()
     t = Timer()
     t.start(stream)
@@ -640,7 +641,7 @@ def main():
                          "empty to skip")
     ap.add_argument("--serving-seconds", type=float, default=60.0)
     ap.add_argument("--only", choices=["roofline", "plain-mix", "batch-mix", "served-mix",
-                                       "ceilings"],
+                                       "ceilings", "default-earth", "render"],
                     help="run one measurement alone (for profiler runs) and print its JSON")
     ap.add_argument("--no-autotune", action="store_true",
                     help="with --only: keep the default block shapes (shorter counter runs)")
@@ -684,6 +685,8 @@ def main():
     if not (args.only and args.no_autotune):
         log("[bench] autotune")
         for name, res in RESOLUTIONS.items():
+            if args.only in ("default-earth", "render") and name != "default":
+                continue  # these modes build default-size sets only
             tuning[name] = b.autotune(Parameters().replace(**res, flags=precise), stream)
 
     if args.only:
@@ -699,6 +702,23 @@ def main():
                    "trace_last": sum(kernels_per_set(q) for q in mix)}
         elif args.only == "batch-mix":
             out = {"batched_ms": batched(b, stream, mix, reps=1)}
+        elif args.only == "default-earth":
+            # The default Earth set relaunched: precision mode, FP32 mode,
+            # binary16 storage (as in the full run's side measurement).
+            out = {}
+            p = Parameters().replace(flags=precise)
+            b.autotune(p.replace(flags=0), stream)
+            for key, q in (("precise", p), ("fp32", p.replace(flags=0)),
+                           ("f16_storage", p.replace(flags=precise | abi.FB_FLAG_F16_STORAGE))):
+                pa, ms = measure_throughput(b, stream, q, args.plain_steps, 5)
+                out[key + "_ms_per_set"] = ms / args.plain_steps
+                pa.close()
+        elif args.only == "render":
+            pa = Atmosphere.build(b, stream, Parameters().replace(flags=precise))
+            pa.wait()
+            mpx, rms = measure_render(b, stream, pa.atmosphere())
+            out = {"mpixel_s": mpx, "ms_per_64_frames": rms}
+            pa.close()
         else:
             wall, lat, failed = serve_all_at_once(b, mix)
             out = {"wall_s": wall, "failed": failed}

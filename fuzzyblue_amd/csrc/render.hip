@@ -429,6 +429,13 @@ __host__ __device__ inline void ray_direction(uint64_t seed, uint32_t frame, uin
 
 // Pixels per thread of the frame kernels (a block shares one slice).
 constexpr int kFramePixelsPerThread = 16;
+// Blocks of 256 threads that cover a frame's pixels.
+__host__ __device__ inline unsigned frame_blocks(uint32_t pixels) {
+    const uint32_t per = 256u * kFramePixelsPerThread;
+    return (pixels + per - 1) / per;
+}
+// Pixel blocks per block of the per-ray frame kernel.
+constexpr unsigned kPerRayStride = 8;
 
 // The camera of a synthetic frame at altitude alt: on the z axis, at
 // radius r.  A frame takes the slice path when the LUTs' slice fits in LDS
@@ -543,10 +550,17 @@ __global__ __launch_bounds__(256) void k_render_frames(DevParams P, const float4
                                                        float4* __restrict__ rad_out,
                                                        float4* __restrict__ trans_out) {
     const uint32_t f = blockIdx.y;
-    render_frame_block<H, SLICE>(P, T, S, altitudes[f], zeniths[f], f, pixels, seed,
-                                 blockIdx.x * blockDim.x * kFramePixelsPerThread, slice != 0,
-                                 slices ? slices + f * sky_slice_float4s(P) : nullptr, rad_out,
-                                 trans_out);
+    // The per-ray kernel walks its frame's pixel blocks with a stride (its
+    // grid is kPerRayStride times smaller), so that a launch whose frames
+    // all take the slice path costs a few hundred blocks, not one per
+    // pixel block.
+    const uint32_t nb = SLICE ? gridDim.x : frame_blocks(pixels);
+    for (uint32_t bx = blockIdx.x; bx < nb; bx += gridDim.x) {  // no LDS on the per-ray path
+        render_frame_block<H, SLICE>(P, T, S, altitudes[f], zeniths[f], f, pixels, seed,
+                                     bx * blockDim.x * kFramePixelsPerThread, slice != 0,
+                                     slices ? slices + f * sky_slice_float4s(P) : nullptr,
+                                     rad_out, trans_out);
+    }
 }
 
 // render_lighting.h:10-28
@@ -584,11 +598,7 @@ __global__ __launch_bounds__(256) void k_render_frame1(DevParams P, const float4
                                  nullptr, rad_out, nullptr);
 }
 
-// Blocks and dynamic LDS of a frame launch.
-static unsigned frame_blocks(uint32_t pixels) {
-    const uint32_t per = 256u * kFramePixelsPerThread;
-    return (pixels + per - 1) / per;
-}
+// Dynamic LDS of a frame launch.
 static size_t frame_lds(const DevParams& P) {
     const size_t b = sky_slice_bytes(P);
     return b <= kSkySliceMaxBytes ? b : 0;
@@ -757,6 +767,7 @@ fb_status fb_render_sky_frames(fb_builder* b, fb_stream stream, const fb_atmosph
                                                                altitudes_km, sun_zenith_rad, \
                                                                pixels, seed, slice, slices, r4, t4)
     if (slice) h ? FB_FRAMES(true, true) : FB_FRAMES(false, true);
+    grid.x = (grid.x + fbk::kPerRayStride - 1) / fbk::kPerRayStride;
     h ? FB_FRAMES(true, false) : FB_FRAMES(false, false);
 #undef FB_FRAMES
     if (slices) {
