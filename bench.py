@@ -271,10 +271,13 @@ def serve_all_at_once(b, ps, budget_gb=200.0, max_inflight=64):
     seconds from the submission to the last completion, per-request
     latencies in ms, failures).  One precompute stream beside the render
     stream: batches admitted as members complete run back to back instead of
-    sharing the device (and its L2) with the batch before them."""
+    sharing the device (and its L2) with the batch before them; while
+    precomputes are in flight the next group is admitted once half of the
+    slots are free (batches of at least 32 rather than one per completion)."""
     from fuzzyblue_amd.serving import Server
 
-    srv = Server(b, int(budget_gb * 1e9), max_inflight=max_inflight, n_streams=2)
+    srv = Server(b, int(budget_gb * 1e9), max_inflight=max_inflight, n_streams=2,
+                 admit_min_free=max_inflight // 2)
     synchronize()
     t0 = time.perf_counter()
     srv.submit_precomputes(ps)

@@ -184,6 +184,7 @@ class fb_server_config(C.Structure):
         ("n_streams", C.c_uint32),
         ("keep_outputs", C.c_uint32),
         ("precompute_share_pct", C.c_uint32),
+        ("admit_min_free", C.c_uint32),
     ]
 
 

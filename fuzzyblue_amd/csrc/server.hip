@@ -315,6 +315,10 @@ struct fb_server {
         size_t slots = cfg.max_inflight;
         for (Req* r : inflight)
             if (r->kind == FB_REQUEST_PRECOMPUTE && slots) --slots;
+        // Hysteresis (admit_min_free): with precomputes in flight, wait for
+        // enough free slots to make the next group a batch worth launching.
+        if (slots < cfg.max_inflight && slots < std::min(cfg.admit_min_free, cfg.max_inflight))
+            return;
         std::vector<Req*> group[2];
         std::deque<Req*> keep;
         bool blocked = false;

@@ -22,9 +22,11 @@ from .workloads import Request
 
 class Server:
     def __init__(self, builder: Builder, hbm_budget_bytes: int, max_inflight: int = 8,
-                 n_streams: int = 4, keep_outputs: bool = False, precompute_share_pct: int = 0):
+                 n_streams: int = 4, keep_outputs: bool = False, precompute_share_pct: int = 0,
+                 admit_min_free: int = 0):
         cfg = abi.fb_server_config(int(hbm_budget_bytes), int(max_inflight), int(n_streams),
-                                   1 if keep_outputs else 0, int(precompute_share_pct))
+                                   1 if keep_outputs else 0, int(precompute_share_pct),
+                                   int(admit_min_free))
         self.handle = C.c_void_p()
         check(load().fb_server_create(builder.handle, C.byref(cfg), C.byref(self.handle)))
         self.builder = builder

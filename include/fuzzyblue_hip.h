@@ -508,6 +508,13 @@ typedef struct fb_server_config {
                                   the device's resident-block slots, so renders
                                   find free slots on every CU; 0 or >= 100: one
                                   block per tile (full occupancy) */
+    uint32_t admit_min_free;   /* 0 or 1: admit precomputes as soon as a slot
+                                  frees (least queueing); k > 1: while
+                                  precomputes are in flight, admit the next
+                                  group only once k slots are free, so a bulk
+                                  submission runs as batches of at least k
+                                  instead of one small batch per completion
+                                  (an idle server admits at once) */
 } fb_server_config;
 
 typedef enum fb_request_kind {

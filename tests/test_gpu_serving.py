@@ -197,3 +197,33 @@ def test_bulk_submission(builder):
         s.submit_precomputes([ps[1], bad])
     assert s.stats().submitted == before
     s.close()
+
+
+@pytest.mark.parametrize("admit_min_free", [3, 64])
+def test_admission_hysteresis(builder, admit_min_free):
+    """admit_min_free: with precomputes in flight the next group waits for
+    that many free slots (64 > max_inflight: only when all are free); every
+    request still completes, in more than one admission when the submission
+    exceeds the slots, and the sets equal plain builds."""
+    rng = np.random.default_rng(12)
+    ps = [random_atmosphere(rng, True, resolution="half") for _ in range(13)]
+    print(f"admit_min_free={admit_min_free}: 13 half sets, 4 slots", flush=True)
+    s = Server(builder, 8 * GB, max_inflight=4, keep_outputs=True, n_streams=2,
+               admit_min_free=admit_min_free)
+    ids = s.submit_precomputes(ps)
+    s.drain()
+    infos = s.poll()
+    assert len(infos) == len(ps) and all(i.status == abi.FB_OK for i in infos)
+    starts = sorted({round(i.t_start, 6) for i in infos})
+    assert len(starts) >= 4  # 13 requests through 4 slots
+    for p, (_, lut) in zip(ps[:1] + ps[-1:], ids[:1] + ids[-1:]):
+        pa = Atmosphere.build(builder, None, p)
+        pa.wait()
+        a = pa.assert_ready()
+        got = fetch(s, lut, p)
+        for k, v in got.items():
+            assert np.array_equal(v, a.download(
+                {abi.FB_LUT_TRANSMITTANCE: "transmittance", abi.FB_LUT_IRRADIANCE: "irradiance",
+                 abi.FB_LUT_SCATTERING: "scattering"}[k])), k
+        a.close()
+    s.close()
