@@ -91,8 +91,10 @@ std::vector<int> tile_order(const std::vector<fb_parameters>& ps) {
 }
 
 // Texels per thread of a multiple-scattering row of W texels at `threads`
-// threads per block (the kernel variants 1, 2, 4).
+// threads per block (the kernel variants 1, 2, 4), or kMsChunkedTpt for the
+// chunked kernel of narrow rows.
 unsigned ms_tpt(const fbk::DevParams& P, unsigned threads) {
+    if (fbk::ms_chunked(P)) return fbk::kMsChunkedTpt;
     const unsigned W = unsigned(P.s_nu * P.s_mus);
     return W <= threads ? 1u : (W <= 2 * threads ? 2u : 4u);
 }
@@ -188,7 +190,9 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
             else
                 g->members.push_back(i);
         } else if (pass == fbk::kPassMultiple) {
-            const unsigned t = fbk::tuned_threads(dps[i], pass, order, tus[i]);
+            const unsigned t = fbk::ms_chunked(dps[i])
+                                   ? fbk::ms_chunk_threads(dps[i])
+                                   : fbk::tuned_threads(dps[i], pass, order, tus[i]);
             const unsigned c = ms_tpt(dps[i], t);
             auto g = std::find_if(groups.begin(), groups.end(),
                                   [&](const Group& x) { return x.threads == t && x.tpt == c; });

@@ -103,6 +103,42 @@ constexpr int kSsTexelsPerThread = 4;
 // directions in sequence.
 constexpr size_t kDenSplitTexels = 65536;
 constexpr int kSsMaxStagedMu = 2048;
+// Multiple scattering over rows of at most kMsChunkMaxW texels (small LUT
+// sets, whose launches are a few hundred one-wavefront blocks, each walking
+// 51 barrier-separated samples): a block of ms_chunk_threads() threads splits
+// its row's samples into ms_chunks() contiguous chunks, one per group of W
+// threads, each integrated with its own staged rows and row slots, and adds
+// the chunk sums in chunk order.  Plain and batched builds choose it alike (by the row
+// width), so a batch member stays bit-identical to a plain build.  Blocks
+// of 256 threads (16 chunks of 3-4 samples at W = 16): one chunk per sample
+// in up to 1024 threads measured no faster.  Rows of 64 texels (the
+// half-resolution class) stay unchunked: chunked, their plain launches ran
+// 1.25x faster, but the 256-atmosphere mix served 0.7% and 256 served
+// half-resolution sets 2.5% slower (batched launches of many sets are not
+// latency-bound, and chunks add row loads).
+constexpr int kMsChunkMaxW = 32;
+constexpr unsigned kMsChunkMaxThreads = 256;
+__host__ __device__ inline bool ms_chunked(const DevParams& P) { return P.s_nu * P.s_mus <= kMsChunkMaxW; }
+__host__ __device__ inline unsigned ms_chunk_threads(const DevParams& P) {
+    const unsigned t = (unsigned(kMsSamples * P.s_nu * P.s_mus) + 63u) & ~63u;
+    return t < kMsChunkMaxThreads ? t : kMsChunkMaxThreads;
+}
+__host__ __device__ inline int ms_chunks(const DevParams& P) {
+    const int c = int(ms_chunk_threads(P)) / (P.s_nu * P.s_mus);
+    return c < kMsSamples ? c : kMsSamples;
+}
+// The texels-per-thread code of a batched multiple-scattering launch that
+// runs the chunked kernel.
+constexpr unsigned kMsChunkedTpt = 0;
+// The row's table and the rows held by the four slots at each sample
+// (float4 each), then staged rows (two per chunk) and the chunks' partial
+// sums, 12-byte rgb.
+__host__ __device__ inline size_t ms_chunked_lds_bytes(const DevParams& P) {
+    const size_t ch = size_t(ms_chunks(P));
+    const size_t wp = size_t(P.s_nu + 1) * size_t(P.s_mus + 1);
+    return 16 * size_t(kMsRowFloat4s + kMsSamples) +
+           12 * (ch * 2 * wp + ch * size_t(P.s_nu * P.s_mus));
+}
 inline bool ss_staged(const DevParams& P) {
     return P.t_mu <= kSsMaxStagedMu && 2 * P.t_mu <= 3 * P.s_nu * P.s_mus;
 }

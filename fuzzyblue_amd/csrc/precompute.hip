@@ -1378,6 +1378,188 @@ __device__ __forceinline__ void multiple_scattering_block(const DevParams& P, co
 }
 
 
+// Rows of at most kMsChunkMaxW texels (ms_chunked): the block's
+// ms_chunks(P) groups of W threads each integrate a contiguous chunk of the
+// row's samples -- per sample, the arithmetic of multiple_scattering_block
+// with TPT = 1, on the group's own staged rows and row slots -- and the
+// chunk sums are added in chunk order.  The row's table is staged in LDS
+// (its reads depend on the group's sample).
+template <typename G, bool H>
+__device__ __forceinline__ void multiple_scattering_block_chunked(
+    const DevParams& P, const int y, const int z, const float4* __restrict__ MR,
+    const float4* __restrict__ J, float4* __restrict__ dMS, float4* __restrict__ S) {
+    // The row table [kMsRowFloat4s] and the slots' rows at each sample
+    // [kMsSamples], then [chunk][2][(NU + 1) * (MUS + 1)] staged rows, then
+    // [chunk][W] sums.
+    extern __shared__ float4 s_ms[];
+    float4* s_desc = s_ms;
+    int4* s_held = reinterpret_cast<int4*>(s_ms + kMsRowFloat4s);
+    Rgb* s_row = reinterpret_cast<Rgb*>(s_ms + kMsRowFloat4s + kMsSamples);
+    const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
+    const int W = NU * MUS;
+    const int CH = ms_chunks(P);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int g = tid / W, x = tid - g * W;
+    const bool in_chunk = g < CH;  // threads past CH * W only take the barriers
+    const int i0 = in_chunk ? g * kMsSamples / CH : 0;
+    const int i1 = in_chunk ? (g + 1) * kMsSamples / CH : 0;
+    const int steps = (kMsSamples + CH - 1) / CH;  // the longest chunk
+
+    const float4* desc = MR + (size_t(z) * MU + y) * kMsRowFloat4s;
+    for (int e = tid; e < kMsRowFloat4s; e += nt) s_desc[e] = desc[e];
+    auto d_tw = [&](int i) { return s_desc[i]; };
+    auto d_geo = [&](int i) { return s_desc[kMsSamples + i]; };
+    auto d_off = [&](int i) { return reinterpret_cast<const int4*>(s_desc + 2 * kMsSamples)[i]; };
+    auto d_wt = [&](int i) { return s_desc[3 * kMsSamples + i]; };
+    const G r = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z)).r;
+    const float rf = float(r);
+
+    float tmu_s = 0.0f, tnu = 0.0f, tslice = 0.0f, tlerp = 0.0f;
+    if (in_chunk) {
+        TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
+        tmu_s = float(c.mu_s);
+        tnu = float(c.nu);
+        float u_nu = (tnu + 1.0f) / 2.0f;
+        float tcx = u_nu * float(NU - 1);
+        tslice = floorf(tcx);
+        tlerp = tcx - tslice;
+    }
+    const int spos = x + x / MUS;
+
+    const float b = P.bottom, b2 = P.bottom * P.bottom;
+    const float H2 = P.top * P.top - P.bottom * P.bottom;
+    const float d_min = P.top - P.bottom;
+    const float inv_range = 1.0f / (sqrtf(H2) - d_min);
+    const float invA = 1.0f / (-2.0f * P.mu_s_min * P.bottom * inv_range);
+    const int SS = MUS + 1;
+    const float musm1 = float(MUS - 1), fSS = float(SS);
+
+    typedef typename std::conditional<H, Half4, Rgb>::type Slot;
+    Slot slot[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) slot[k] = Slot{};
+    constexpr int TB = H ? 8 : 16;
+    const __amdgpu_buffer_rsrc_t jb = buffer_rsrc(J, int(size_t(P.s_r) * MU * W * TB));
+    const int xoff = x * TB;
+    auto ld = [&](int o) {
+        if constexpr (H) {
+            return __builtin_bit_cast(Half4, __builtin_amdgcn_raw_buffer_load_b64(jb, xoff, o * TB, 0));
+        } else {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b96(jb, xoff, o * TB, 0);
+            return Rgb{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2])};
+        }
+    };
+    auto advance = [&](int mask, int4 o) {
+        if (mask & 1) slot[0] = ld(o.x);
+        if (mask & 2) slot[1] = ld(o.y);
+        if (mask & 4) slot[2] = ld(o.z);
+        if (mask & 8) slot[3] = ld(o.w);
+    };
+    // Each chunk's two staged rows, laid out as in multiple_scattering_block
+    // (pad entry per nu slice, zero slice NU).
+    const int WP = (NU + 1) * SS;
+    const int PADS = NU + 1 + MUS;
+    for (int e = tid; e < CH * PADS; e += nt) {
+        const int cg = e / PADS, e0 = e - cg * PADS;
+        const int k = e0 < NU ? e0 * SS + MUS : NU * SS + (e0 - NU);
+        s_row[(2 * cg) * WP + k] = Rgb{0.0f, 0.0f, 0.0f};
+        s_row[(2 * cg + 1) * WP + k] = Rgb{0.0f, 0.0f, 0.0f};
+    }
+    Rgb* rows = s_row + 2 * (in_chunk ? g : 0) * WP;
+    auto put = [&](int i, Rgb* dst) {
+        const float4 w = d_wt(i);
+        const Rgb t[4] = {to_rgb(slot[0]), to_rgb(slot[1]), to_rgb(slot[2]), to_rgb(slot[3])};
+        dst[spos] = Rgb{fmaf(w.w, t[3].x, fmaf(w.z, t[2].x, fmaf(w.y, t[1].x, w.x * t[0].x))),
+                        fmaf(w.w, t[3].y, fmaf(w.z, t[2].y, fmaf(w.y, t[1].y, w.x * t[0].y))),
+                        fmaf(w.w, t[3].z, fmaf(w.z, t[2].z, fmaf(w.y, t[1].z, w.x * t[0].z)))};
+    };
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    auto eval = [&](int i, const Rgb* __restrict__ cur) {
+        const float4 tw = d_tw(i);
+        const float inv_r_i = d_geo(i).x, d_i = tw.w;
+        float mu_s_i = fmaf(rf, tmu_s, d_i * tnu) * inv_r_i;
+        mu_s_i = __builtin_amdgcn_fmed3f(mu_s_i, -1.0f, 1.0f);
+        float disc = fmaf(b2, mu_s_i * mu_s_i, H2);
+        float d = fmaxf(fmaf(-b, mu_s_i, __builtin_amdgcn_sqrtf(disc)), 0.0f);
+        float a = (d - d_min) * inv_range;
+        float xm = __builtin_amdgcn_fmed3f(
+            fmaf(-a, invA, 1.0f) * __builtin_amdgcn_rcpf(1.0f + a), 0.0f, 1.0f);
+        float xt = fmaf(xm, musm1, tslice * fSS);
+        float fl = floorf(xt);
+        float w = xt - fl;
+        int k0 = int(fl);
+        const Rgb p0 = cur[k0], p1 = cur[k0 + 1], p2 = cur[k0 + SS], p3 = cur[k0 + SS + 1];
+        float l = tlerp;
+        float v0 = fmaf(w, p1.x - p0.x, p0.x), v1 = fmaf(w, p1.y - p0.y, p0.y),
+              v2 = fmaf(w, p1.z - p0.z, p0.z);
+        float u0 = fmaf(w, p3.x - p2.x, p2.x), u1 = fmaf(w, p3.y - p2.y, p2.y),
+              u2 = fmaf(w, p3.z - p2.z, p2.z);
+        a0 = a0 + fmaf(l, u0 - v0, v0) * tw.x;
+        a1 = a1 + fmaf(l, u1 - v1, v1) * tw.y;
+        a2 = a2 + fmaf(l, u2 - v2, v2) * tw.z;
+    };
+    __syncthreads();  // the table and the rows' pads
+    // The rows the four slots hold at each sample: a slot that a sample uses
+    // without loading it holds the row it was last loaded with (the table's
+    // offset is 0 for slots a sample does not load).  Lane j of the first
+    // wavefront takes sample j: per slot, the last sample <= j that loads
+    // it, by a prefix maximum over the lanes.
+    if (tid < 64) {
+        const int j = tid;
+        const int m = j < kMsSamples ? __float_as_int(d_geo(j).y) : 0;
+        int last[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int v = (m >> t) & 1 ? j : -1;
+#pragma unroll
+            for (int k = 1; k < 64; k <<= 1) {
+                const int u = __shfl_up(v, k, 64);
+                if (j >= k) v = max(v, u);
+            }
+            last[t] = v;
+        }
+        if (j < kMsSamples)
+            s_held[j] = make_int4(last[0] >= 0 ? d_off(last[0]).x : 0,
+                                  last[1] >= 0 ? d_off(last[1]).y : 0,
+                                  last[2] >= 0 ? d_off(last[2]).z : 0,
+                                  last[3] >= 0 ? d_off(last[3]).w : 0);
+    }
+    __syncthreads();
+    if (in_chunk) {
+        advance(15, s_held[i0]);
+        put(i0, rows);
+    }
+    __syncthreads();
+    for (int s = 0; s < steps; ++s) {
+        const int i = i0 + s;
+        const bool has = in_chunk && i < i1;
+        const bool next = in_chunk && i + 1 < i1;
+        if (next) advance(__float_as_int(d_geo(i + 1).y), d_off(i + 1));
+        if (has) eval(i, rows + (s & 1) * WP);
+        if (next) put(i + 1, rows + ((s + 1) & 1) * WP);
+        __syncthreads();
+    }
+    Rgb* part = s_row + 2 * CH * WP;
+    if (in_chunk) part[g * W + x] = Rgb{a0, a1, a2};
+    __syncthreads();
+    if (g == 0) {
+        float m0 = part[x].x, m1 = part[x].y, m2 = part[x].z;
+        for (int c = 1; c < CH; ++c) {
+            const Rgb q = part[c * W + x];
+            m0 = m0 + q.x;
+            m1 = m1 + q.y;
+            m2 = m2 + q.z;
+        }
+        const size_t idx = (size_t(z) * MU + y) * W + x;
+        st_tex(dMS, idx, make_float4(m0, m1, m2, 0.0f), H);
+        float pr = phase_rayleigh(tnu);
+        float4 prev = ld_tex(S, idx, H);
+        st_tex(S, idx,
+               make_float4(m0 / pr + prev.x, m1 / pr + prev.y, m2 / pr + prev.z, 0.0f + prev.w),
+               H);
+    }
+}
+
 // ================================================== kernel entry points
 // Plain kernels: one atmosphere, block coordinates from blockIdx, the
 // parameters as a kernel argument.  Batched kernels (kb_*): one launch over
@@ -1608,6 +1790,28 @@ __global__ __launch_bounds__(kMsMaxThreads) void kb_multiple_scattering(const De
     }
 }
 
+template <typename G, bool H>
+__global__ __launch_bounds__(kMsChunkMaxThreads) void k_multiple_scattering_chunked(
+    DevParams P, const float4* __restrict__ MR, const float4* __restrict__ J,
+    float4* __restrict__ dMS, float4* __restrict__ S, unsigned remap) {
+    const unsigned b = logical_block(remap);
+    multiple_scattering_block_chunked<G, H>(P, b % P.s_mu, b / P.s_mu, MR, J, dMS, S);
+}
+template <typename G, bool H>
+__global__ __launch_bounds__(kMsChunkMaxThreads) void kb_multiple_scattering_chunked(
+    const DevParams* __restrict__ Ps, const LutPtrs* __restrict__ L,
+    const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const DevParams& P = Ps[atm];
+        const LutPtrs l = lut_ptrs(L, atm);
+        multiple_scattering_block_chunked<G, H>(P, local % P.s_mu, local / P.s_mu, l.MR, l.J,
+                                                l.dMS, l.S);
+    }
+}
+
 // ================================================================== clear
 __global__ void k_fill_zero(float4* __restrict__ p, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1761,8 +1965,19 @@ static hipError_t launch_multiple_scattering_h(const DevParams& P, const float4*
     const int W = P.s_nu * P.s_mus;
     const int threads = int(tuned_threads(P, kPassMultiple, 2, tu));
     const unsigned remap = (tu.remap >> kPassMultiple) & 1u;
-    if (W > threads * kMsTexelsPerThread) return hipErrorInvalidValue;
     dim3 grid(P.s_mu, P.s_r);
+    if (ms_chunked(P)) {
+        const size_t lds = ms_chunked_lds_bytes(P);
+        const unsigned ct = ms_chunk_threads(P);
+        if (P.flags & 1)
+            k_multiple_scattering_chunked<double, H><<<grid, ct, lds, s>>>(P, MR, J, dMS, S,
+                                                                          remap);
+        else
+            k_multiple_scattering_chunked<float, H><<<grid, ct, lds, s>>>(P, MR, J, dMS, S,
+                                                                         remap);
+        return hipGetLastError();
+    }
+    if (W > threads * kMsTexelsPerThread) return hipErrorInvalidValue;
     size_t lds = sizeof(Rgb) * kMsStagedRows * size_t(P.s_nu + 1) * (P.s_mus + 1);
     // Texels per thread: 1 up to 256 texels per row, else 2 or 4.  The
     // arithmetic of a texel does not depend on it.
@@ -1865,6 +2080,9 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
                     indirect_lds_bytes_per_wave(P.s_nu, (order - 1) == 1) * waves};
         }
         case kPassMultiple:
+            if (ms_chunked(P))
+                return {unsigned(P.s_mu) * unsigned(P.s_r), ms_chunk_threads(P),
+                        ms_chunked_lds_bytes(P)};
             return {unsigned(P.s_mu) * unsigned(P.s_r), threads,
                     sizeof(Rgb) * kMsStagedRows * size_t(P.s_nu + 1) * (P.s_mus + 1)};
     }
@@ -1934,7 +2152,12 @@ static hipError_t launch_batch_h(int pass, int order, bool precise, const DevPar
             break;
         case kPassMultiple:
 #define FB_KBMS(G, T) kb_multiple_scattering<G, false, T, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles)
-            if (precise)
+            if (tpt == kMsChunkedTpt) {
+                if (precise)
+                    kb_multiple_scattering_chunked<double, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                else
+                    kb_multiple_scattering_chunked<float, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+            } else if (precise)
                 tpt == 1 ? FB_KBMS(double, 1) : tpt == 2 ? FB_KBMS(double, 2) : FB_KBMS(double, 4);
             else
                 tpt == 1 ? FB_KBMS(float, 1) : tpt == 2 ? FB_KBMS(float, 2) : FB_KBMS(float, 4);
@@ -1998,7 +2221,10 @@ static unsigned batch_resident_blocks_h(int pass, int order, bool precise, unsig
                                    : (const void*)kb_indirect_irradiance<float, false, H>);
             break;
         case kPassMultiple:
-            if (tpt == 1)
+            if (tpt == kMsChunkedTpt)
+                fn = precise ? (const void*)kb_multiple_scattering_chunked<double, H>
+                             : (const void*)kb_multiple_scattering_chunked<float, H>;
+            else if (tpt == 1)
                 fn = precise ? (const void*)kb_multiple_scattering<double, false, 1, H>
                              : (const void*)kb_multiple_scattering<float, false, 1, H>;
             else if (tpt == 2)

@@ -25,6 +25,9 @@ def small_mix(precise):
         random_atmosphere(rng, precise, resolution="half", order=7),
         random_atmosphere(rng, precise, resolution="half", order=2),
         Parameters().replace(**RESOLUTIONS["half"], order=4, ground_albedo=[0.3, 0.2, 0.1]),
+        # rows of 32 texels: chunked multiple scattering, as in a plain build
+        Parameters().replace(scattering_r_size=8, scattering_mu_size=32,
+                             scattering_mu_s_size=8, scattering_nu_size=4, order=4),
     ]
     for p in ps:
         p.flags = abi.FB_FLAG_PRECISE if precise else 0

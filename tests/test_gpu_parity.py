@@ -53,9 +53,17 @@ def scratch(pa):
 SMOKE = Parameters().replace(**SMOKE_SIZES)
 
 
-@pytest.mark.parametrize("p", [SMOKE, Parameters().replace(**DUMP_SIZES)], ids=["smoke", "dump"])
+# Rows of 32 texels: multiple scattering in 8 chunks of 6-7 samples.
+NARROW32 = Parameters().replace(scattering_r_size=8, scattering_mu_size=32,
+                                scattering_mu_s_size=8, scattering_nu_size=4)
+
+
+@pytest.mark.parametrize("p", [SMOKE, NARROW32, Parameters().replace(**DUMP_SIZES)],
+                         ids=["smoke", "narrow32", "dump"])
 def test_passes_fp32_vs_restatement(builder, p):
-    """Each pass on the GPU's own inputs against the FP32 restatement."""
+    """Each pass on the GPU's own inputs against the FP32 restatement (at
+    the smoke and narrow32 sizes multiple scattering runs chunked: its
+    rows are at most 32 texels wide)."""
     o = Oracle("f32")
     print(f"\n[parity fp32] sizes {p.scattering_extent()}", flush=True)
     # order 1: transmittance, direct irradiance, single scattering
