@@ -179,14 +179,18 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
     for (int i : order_idx) {
         if (!member(i)) continue;
         if (pass == fbk::kPassSingle) {
-            // The staged kernel (transmittance rows in LDS) or, for very wide
-            // transmittance LUTs, the direct one: one launch per kernel.
-            const unsigned c = fbk::ss_staged(dps[i]) ? 1u : 0u;
+            // The staged kernel (transmittance rows in LDS), the direct one,
+            // or the chunked direct one for narrow rows: one launch per kernel.
+            const unsigned c = fbk::ss_chunked(dps[i])  ? fbk::kSsChunked
+                               : fbk::ss_staged(dps[i]) ? fbk::kSsStaged
+                                                        : fbk::kSsDirect;
             auto g = std::find_if(groups.begin(), groups.end(),
                                   [&](const Group& x) { return x.tpt == c; });
             if (g == groups.end())
-                groups.push_back(Group{fbk::tuned_threads(dps[i], pass, order, tus[i]), c,
-                                       bool((tus[i].remap >> pass) & 1u), {i}});
+                groups.push_back(Group{c == fbk::kSsChunked
+                                           ? fbk::kSsChunkThreads
+                                           : fbk::tuned_threads(dps[i], pass, order, tus[i]),
+                                       c, bool((tus[i].remap >> pass) & 1u), {i}});
             else
                 g->members.push_back(i);
         } else if (pass == fbk::kPassMultiple) {

@@ -143,6 +143,28 @@ inline bool ss_staged(const DevParams& P) {
     return P.t_mu <= kSsMaxStagedMu && 2 * P.t_mu <= 3 * P.s_nu * P.s_mus;
 }
 inline size_t ss_staged_lds_bytes(int t_mu) { return 2 * size_t(t_mu) * 16; }
+// Unstaged single scattering over rows of at most kSsChunkMaxW texels (small
+// LUT sets): each thread of a one-wavefront block would walk 51 samples,
+// each waiting for its transmittance gathers.  A block of kSsChunkThreads
+// threads instead splits the samples into ss_chunks() contiguous chunks, one
+// per group of W threads, and adds the chunk sums in chunk order (chosen by
+// the sizes alike in plain and batched builds).
+constexpr int kSsChunkMaxW = 32;
+constexpr unsigned kSsChunkThreads = 256;
+__host__ __device__ inline int ss_chunks(const DevParams& P) {
+    const int c = int(kSsChunkThreads) / (P.s_nu * P.s_mus);
+    return c < kMsSamples ? c : kMsSamples;
+}
+inline bool ss_chunked(const DevParams& P) {
+    return !ss_staged(P) && P.s_nu * P.s_mus <= kSsChunkMaxW;
+}
+// Per chunk and texel, the Rayleigh and Mie partial sums (6 floats).
+inline size_t ss_chunked_lds_bytes(const DevParams& P) {
+    return 24 * size_t(ss_chunks(P)) * size_t(P.s_nu * P.s_mus);
+}
+// The kernel code of a batched single-scattering launch: staged, direct
+// lookups, or chunked direct lookups.
+constexpr unsigned kSsDirect = 0, kSsStaged = 1, kSsChunked = 2;
 
 unsigned tuned_threads(const DevParams& P, int pass, int order, const Tuning& tu);
 bool tuning_valid(const DevParams& P, int pass, int order, unsigned threads);

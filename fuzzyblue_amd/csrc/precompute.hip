@@ -590,6 +590,91 @@ __device__ __forceinline__ void single_scattering_block(const DevParams& P, cons
     }
 }
 
+// Rows of at most kSsChunkMaxW texels on the direct-lookup path
+// (ss_chunked): the block's ss_chunks(P) groups of W threads each integrate
+// a contiguous chunk of the row's samples -- per sample, the arithmetic of
+// single_scattering_block's direct path -- and the chunk sums are added in
+// chunk order.  The row table's reads depend on the group's sample: vector
+// loads.
+template <typename G, bool H>
+__device__ __forceinline__ void single_scattering_block_chunked(
+    const DevParams& P, const int y, const int z, const float4* __restrict__ T,
+    const float4* __restrict__ SR, float4* __restrict__ dR, float4* __restrict__ dM,
+    float4* __restrict__ S) {
+    extern __shared__ float s_ssp[];  // [chunk][W][6]: Rayleigh rgb, Mie rgb
+    const float4* row = SR + (size_t(z) * P.s_mu + y) * kSsRowFloat4s;
+    const float4* s_tv = row;
+    const float4* s_geo = row + kMsSamples;
+    const float4* s_lut = row + 2 * kMsSamples;
+    const float4* s_inv = row + 3 * kMsSamples;
+    const float4* s_trw = row + 4 * kMsSamples;
+    constexpr bool kPrecise = sizeof(G) == sizeof(double);
+    const int NU = P.s_nu, MUS = P.s_mus, MU = P.s_mu;
+    const int W = NU * MUS;
+    const int CH = ss_chunks(P);
+    const int tid = threadIdx.x;
+    const int g = tid / W, x = tid - g * W;
+    const bool in_chunk = g < CH;
+    const G r = scattering_texel_coord<G>(P, 0u, unsigned(y), unsigned(z)).r;
+    const float rf = float(r);
+    const float top2 = P.top * P.top;
+    const int TMU = P.t_mu;
+    if (in_chunk) {
+        const int i0 = g * kMsSamples / CH, i1 = (g + 1) * kMsSamples / CH;
+        TexelCoord<G> c = scattering_texel_coord<G>(P, unsigned(x), unsigned(y), unsigned(z));
+        const float mu_s = float(c.mu_s), nu = float(c.nu);
+        float rs0 = 0.0f, rs1 = 0.0f, rs2 = 0.0f, ms0 = 0.0f, ms1 = 0.0f, ms2 = 0.0f;
+        for (int i = i0; i < i1; ++i) {
+            const float4 tv = s_tv[i], gq = s_geo[i], lq = s_lut[i], iv = s_inv[i],
+                         tr4 = s_trw[i];
+            const int row0 = __float_as_int(tr4.x), drow = (__float_as_int(tr4.y) - row0) * 16;
+            const __amdgpu_buffer_rsrc_t tb = buffer_rsrc(T + row0, drow + TMU * 16);
+            float t, sm;
+            ss_sun_coord<kPrecise>(rf, mu_s, nu, tv, gq, lq, iv, top2, TMU, t, sm);
+            const int k0 = int(t);
+            const int o0 = k0 * 16, o1 = min(k0 + 1, TMU - 1) * 16;
+            const float f = t - float(k0);
+            const float3 ts =
+                lerp3(lerp3(buffer_rgb(tb, o0), buffer_rgb(tb, o1), f),
+                      lerp3(buffer_rgb(tb, drow + o0), buffer_rgb(tb, drow + o1), f), lq.w);
+            const float t0 = tv.x * (ts.x * sm), t1 = tv.y * (ts.y * sm), t2 = tv.z * (ts.z * sm);
+            rs0 = rs0 + t0 * gq.y;
+            rs1 = rs1 + t1 * gq.y;
+            rs2 = rs2 + t2 * gq.y;
+            ms0 = ms0 + t0 * gq.z;
+            ms1 = ms1 + t1 * gq.z;
+            ms2 = ms2 + t2 * gq.z;
+        }
+        float* p = s_ssp + size_t(g * W + x) * 6;
+        p[0] = rs0;
+        p[1] = rs1;
+        p[2] = rs2;
+        p[3] = ms0;
+        p[4] = ms1;
+        p[5] = ms2;
+    }
+    __syncthreads();
+    if (g == 0) {
+        float v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = s_ssp[size_t(x) * 6 + k];
+        for (int cg = 1; cg < CH; ++cg)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = v[k] + s_ssp[size_t(cg * W + x) * 6 + k];
+        const float dxf = s_trw[0].z;
+        const float3 ray = make_float3(v[0] * dxf * P.solar[0] * P.rayleigh_scattering[0],
+                                       v[1] * dxf * P.solar[1] * P.rayleigh_scattering[1],
+                                       v[2] * dxf * P.solar[2] * P.rayleigh_scattering[2]);
+        const float3 mie = make_float3(v[3] * dxf * P.solar[0] * P.mie_scattering[0],
+                                       v[4] * dxf * P.solar[1] * P.mie_scattering[1],
+                                       v[5] * dxf * P.solar[2] * P.mie_scattering[2]);
+        const size_t idx = (size_t(z) * MU + y) * W + x;
+        st_tex(dR, idx, make_float4(ray.x, ray.y, ray.z, 0.0f), H);
+        st_tex(dM, idx, make_float4(mie.x, mie.y, mie.z, 0.0f), H);
+        st_tex(S, idx, make_float4(ray.x, ray.y, ray.z, mie.x), H);
+    }
+}
+
 // ===================================================== 4. scattering density
 // scattering_density.comp:10-155.
 //
@@ -1673,6 +1758,28 @@ __global__ __launch_bounds__(256) void kb_single_scattering(const DevParams* __r
     }
 }
 
+template <typename G, bool H>
+__global__ __launch_bounds__(kSsChunkThreads) void k_single_scattering_chunked(
+    DevParams P, const float4* __restrict__ T, const float4* __restrict__ SR,
+    float4* __restrict__ dR, float4* __restrict__ dM, float4* __restrict__ S, unsigned remap) {
+    const unsigned b = logical_block(remap);
+    single_scattering_block_chunked<G, H>(P, b % P.s_mu, b / P.s_mu, T, SR, dR, dM, S);
+}
+template <typename G, bool H>
+__global__ __launch_bounds__(kSsChunkThreads) void kb_single_scattering_chunked(
+    const DevParams* __restrict__ Ps, const LutPtrs* __restrict__ L,
+    const uint32_t* __restrict__ tiles, unsigned ntiles) {
+    for (unsigned t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (t != blockIdx.x) __syncthreads();
+        int atm, local;
+        decode_tile(tiles[t], atm, local);
+        const DevParams& P = Ps[atm];
+        const LutPtrs l = lut_ptrs(L, atm);
+        single_scattering_block_chunked<G, H>(P, local % P.s_mu, local / P.s_mu, l.T, l.SR, l.dR,
+                                              l.dM, l.S);
+    }
+}
+
 template <typename G, bool SINGLE, bool H>
 __global__ __launch_bounds__(256, 8) void k_scattering_density(
     DevParams P, const float4* __restrict__ DG, const float4* __restrict__ dR,
@@ -1874,7 +1981,15 @@ static hipError_t launch_single_scattering_h(const DevParams& P, const float4* T
     const unsigned threads = tuned_threads(P, kPassSingle, 2, tu);
     const unsigned remap = (tu.remap >> kPassSingle) & 1u;
     dim3 grid(P.s_mu, P.s_r);
-    if (ss_staged(P)) {
+    if (ss_chunked(P)) {
+        const size_t lds = ss_chunked_lds_bytes(P);
+        if (P.flags & 1)
+            k_single_scattering_chunked<double, H>
+                <<<grid, kSsChunkThreads, lds, s>>>(P, T, SR, dR, dM, S, remap);
+        else
+            k_single_scattering_chunked<float, H>
+                <<<grid, kSsChunkThreads, lds, s>>>(P, T, SR, dR, dM, S, remap);
+    } else if (ss_staged(P)) {
         const size_t lds = ss_staged_lds_bytes(P.t_mu);
         if (P.flags & 1)
             k_single_scattering<double, false, H, true>
@@ -2064,6 +2179,9 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
         case kPassDirect:
             return {cdiv(P.i_mus, 64) * unsigned(P.i_r), 64, 0};
         case kPassSingle:
+            if (ss_chunked(P))
+                return {unsigned(P.s_mu) * unsigned(P.s_r), kSsChunkThreads,
+                        ss_chunked_lds_bytes(P)};
             return {unsigned(P.s_mu) * unsigned(P.s_r), threads,
                     ss_staged(P) ? ss_staged_lds_bytes(P.t_mu) : 0};
         case kPassClear:
@@ -2111,8 +2229,13 @@ static hipError_t launch_batch_h(int pass, int order, bool precise, const DevPar
             else
                 kb_direct_irradiance<float><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
-        case kPassSingle:  // tpt: 1 for the staged kernel (lds > 0), 0 otherwise
-            if (precise && tpt)
+        case kPassSingle:  // tpt: kSsStaged, kSsDirect or kSsChunked
+            if (tpt == kSsChunked) {
+                if (precise)
+                    kb_single_scattering_chunked<double, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+                else
+                    kb_single_scattering_chunked<float, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+            } else if (precise && tpt)
                 kb_single_scattering<double, false, H, true><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
             else if (precise)
                 kb_single_scattering<double, false, H, false><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
@@ -2195,7 +2318,10 @@ static unsigned batch_resident_blocks_h(int pass, int order, bool precise, unsig
                          : (const void*)kb_direct_irradiance<float>;
             break;
         case kPassSingle:
-            if (tpt)
+            if (tpt == kSsChunked)
+                fn = precise ? (const void*)kb_single_scattering_chunked<double, H>
+                             : (const void*)kb_single_scattering_chunked<float, H>;
+            else if (tpt)
                 fn = precise ? (const void*)kb_single_scattering<double, false, H, true>
                              : (const void*)kb_single_scattering<float, false, H, true>;
             else
