@@ -466,7 +466,10 @@ fb_status fb_render_sky_rays_shadowed(fb_builder *b, fb_stream stream,
  * (0, 0, bottom + altitude[f]) and sun (sin zs[f], 0, cos zs[f]); pixel i of
  * frame f looks along a direction uniform on the sphere drawn from a
  * counter-based hash of (seed, f, i) (fb_ray_direction below).  Writes
- * n_frames*pixels float4 radiance and transmittance (GetSkyRadiance). */
+ * n_frames*pixels float4 radiance and transmittance (GetSkyRadiance).  When
+ * the LUTs' per-frame slice fits a block's LDS (default sizes), the call
+ * takes n_frames * (at most 64 KB) of stream-ordered scratch on `stream`
+ * (hipMallocAsync, freed after its launches); FB_ENOMEM if that fails. */
 fb_status fb_render_sky_frames(fb_builder *b, fb_stream stream,
                                const fb_atmosphere *a,
                                const float *altitudes_km,
