@@ -34,7 +34,9 @@ struct BatchLaunch {
     std::vector<uint32_t> host_tiles;  // until uploaded
     size_t offset = 0;                 // of d_tiles in the plan's device block
     unsigned grid = 0;                 // persistent grid size (0: one block per tile)
-    unsigned tpt = 1;                  // multiple scattering: texels per thread
+    unsigned tpt = 1;                  // kernel variant: multiple scattering texels per
+                                       // thread (or chunked), single scattering kernel,
+                                       // density texels per thread
 };
 
 // Shared state of one batch: device arrays, the launch list and the graph.
@@ -202,6 +204,16 @@ fb_status add_launch(BatchPlan& plan, const std::vector<fb_parameters>& ps,
                                   [&](const Group& x) { return x.threads == t && x.tpt == c; });
             if (g == groups.end())
                 groups.push_back(Group{t, c, bool((tus[i].remap >> pass) & 1u), {i}});
+            else
+                g->members.push_back(i);
+        } else if (pass == fbk::kPassDensity) {
+            // One launch per texels-per-thread variant (den_tpx).
+            const unsigned c = unsigned(fbk::den_tpx(dps[i]));
+            auto g = std::find_if(groups.begin(), groups.end(),
+                                  [&](const Group& x) { return x.tpt == c; });
+            if (g == groups.end())
+                groups.push_back(Group{fbk::tuned_threads(dps[i], pass, order, tus[i]), c,
+                                       bool((tus[i].remap >> pass) & 1u), {i}});
             else
                 g->members.push_back(i);
         } else if (groups.empty()) {

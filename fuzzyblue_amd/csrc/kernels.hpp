@@ -102,6 +102,20 @@ constexpr int kSsTexelsPerThread = 4;
 // texel would leave most of the device idle, each thread walking all 512
 // directions in sequence.
 constexpr size_t kDenSplitTexels = 65536;
+// Density texels per thread: 2 when the LUT set is not theta-split, NU is
+// even and an (r, mu_s) tile holds at least kDenPairMinTexels texels — the
+// thread then takes texels (mu, nu) and (mu, nu + NU/2), which share
+// nu2 = omega.omega_i, q and the Cornette-Shanks power of every direction
+// pair (computed once for both; each texel's own operations and their order
+// are unchanged, so the values are the same bits), and a block's profile
+// build serves twice the texels.  At most kDenPairOccupancy blocks of 256
+// threads per CU then (the pair's registers).
+constexpr int kDenPairMinTexels = 512;
+constexpr int kDenPairOccupancy = 5;
+__host__ __device__ inline int den_tpx(const DevParams& P) {
+    return (P.den_split == 1 && P.s_nu % 2 == 0 && P.s_mu * P.s_nu >= kDenPairMinTexels) ? 2
+                                                                                          : 1;
+}
 constexpr int kSsMaxStagedMu = 2048;
 // Multiple scattering over rows of at most kMsChunkMaxW texels (small LUT
 // sets, whose launches are a few hundred one-wavefront blocks, each walking

@@ -831,6 +831,60 @@ __device__ __forceinline__ void pair_lerp_i(const PairEntry* __restrict__ prof, 
 // kept apart (aR = sum of q L, aM = sum of q w L, with q = 1 + nu2^2 and w
 // the Cornette-Shanks power): their scattering coefficients are constant
 // over the row and are applied once by the caller.
+// One direction pair of density_row: the lookups of phi_m and phi_{31-m}
+// for one texel, added to its row sums with the pair's q and q w.
+template <bool SINGLE, bool GROUND>
+__device__ __forceinline__ void density_pair(const DenTexel& dt, const DenRow& row, float cx,
+                                             float sy, float q, float qw, float3& aR,
+                                             float3& aM) {
+    float3 La, Lb;
+    if (SINGLE) {
+        const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
+        const float nua = base + sb, nub = base - sb;
+        // entry coordinates (nu + 1) (NU-1)/2 + 1
+        const float ta = fmaf(nua, dt.hnu, dt.hnu1), tb = fmaf(nub, dt.hnu, dt.hnu1);
+        // Profiles are pre-scaled by kR (rayleigh) and kM (mie).
+        float3 Ma, Mb;
+        pair_lerp_i(row.pp, ta, La, Ma);
+        pair_lerp_i(row.pp, tb, Lb, Mb);
+        const float qa = fmaf(nua, nua, 1.0f), qb = fmaf(nub, nub, 1.0f);
+        const float wa = rsqrt_cubed(fmaf(dt.g2, nua, dt.g1));
+        const float wb = rsqrt_cubed(fmaf(dt.g2, nub, dt.g1));
+        La = make_float3(qa * fmaf(Ma.x, wa, La.x), qa * fmaf(Ma.y, wa, La.y),
+                         qa * fmaf(Ma.z, wa, La.z));
+        Lb = make_float3(qb * fmaf(Mb.x, wb, Lb.x), qb * fmaf(Mb.y, wb, Lb.y),
+                         qb * fmaf(Mb.z, wb, Lb.z));
+    } else {
+        // entry coordinate (nu + 1) (NU-1)/2 + 1 with the scale and
+        // offset folded into the per-texel and per-row terms
+        const float base = fmaf(cx, dt.osxh, row.Ch1), sb = sy * dt.osyh;
+        La = prof_lerp_i(row.pr, base + sb);
+        Lb = prof_lerp_i(row.pr, base - sb);
+    }
+    if (GROUND) {
+        const float gb = fmaf(cx, row.Ag, row.Cg1), gs = sy * row.Bg;
+        const float3 Ea = prof_lerp_i(dt.erow, gb + gs);
+        const float3 Eb = prof_lerp_i(dt.erow, gb - gs);
+        La = make_float3(fmaf(row.gv.x, Ea.x, La.x), fmaf(row.gv.y, Ea.y, La.y),
+                         fmaf(row.gv.z, Ea.z, La.z));
+        Lb = make_float3(fmaf(row.gv.x, Eb.x, Lb.x), fmaf(row.gv.y, Eb.y, Lb.y),
+                         fmaf(row.gv.z, Eb.z, Lb.z));
+    }
+    const float Sx = La.x + Lb.x, Sy = La.y + Lb.y, Sz = La.z + Lb.z;
+    aR.x = fmaf(q, Sx, aR.x);
+    aR.y = fmaf(q, Sy, aR.y);
+    aR.z = fmaf(q, Sz, aR.z);
+    aM.x = fmaf(qw, Sx, aM.x);
+    aM.y = fmaf(qw, Sy, aM.y);
+    aM.z = fmaf(qw, Sz, aM.z);
+}
+
+// The 16 direction pairs (phi_m, phi_{31-m}) of one theta row.  Both members
+// share cos(phi), hence nu2 = omega.omega_i and the phase functions; their
+// sin(phi) terms have opposite signs.  The row's Rayleigh and Mie sums are
+// kept apart (aR = sum of q L, aM = sum of q w L, with q = 1 + nu2^2 and w
+// the Cornette-Shanks power): their scattering coefficients are constant
+// over the row and are applied once by the caller.
 template <bool SINGLE, bool GROUND>
 __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& row, float3& aR,
                                             float3& aM) {
@@ -841,46 +895,25 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
         const float nu2 = fmaf(cx, dt.omx, row.omzc);
         const float q = fmaf(nu2, nu2, 1.0f);
         const float qw = q * rsqrt_cubed(fmaf(dt.g2, nu2, dt.g1));
-        float3 La, Lb;
-        if (SINGLE) {
-            const float base = fmaf(cx, dt.osx, row.C), sb = sy * dt.osy;
-            const float nua = base + sb, nub = base - sb;
-            // entry coordinates (nu + 1) (NU-1)/2 + 1
-            const float ta = fmaf(nua, dt.hnu, dt.hnu1), tb = fmaf(nub, dt.hnu, dt.hnu1);
-            // Profiles are pre-scaled by kR (rayleigh) and kM (mie).
-            float3 Ma, Mb;
-            pair_lerp_i(row.pp, ta, La, Ma);
-            pair_lerp_i(row.pp, tb, Lb, Mb);
-            const float qa = fmaf(nua, nua, 1.0f), qb = fmaf(nub, nub, 1.0f);
-            const float wa = rsqrt_cubed(fmaf(dt.g2, nua, dt.g1));
-            const float wb = rsqrt_cubed(fmaf(dt.g2, nub, dt.g1));
-            La = make_float3(qa * fmaf(Ma.x, wa, La.x), qa * fmaf(Ma.y, wa, La.y),
-                             qa * fmaf(Ma.z, wa, La.z));
-            Lb = make_float3(qb * fmaf(Mb.x, wb, Lb.x), qb * fmaf(Mb.y, wb, Lb.y),
-                             qb * fmaf(Mb.z, wb, Lb.z));
-        } else {
-            // entry coordinate (nu + 1) (NU-1)/2 + 1 with the scale and
-            // offset folded into the per-texel and per-row terms
-            const float base = fmaf(cx, dt.osxh, row.Ch1), sb = sy * dt.osyh;
-            La = prof_lerp_i(row.pr, base + sb);
-            Lb = prof_lerp_i(row.pr, base - sb);
-        }
-        if (GROUND) {
-            const float gb = fmaf(cx, row.Ag, row.Cg1), gs = sy * row.Bg;
-            const float3 Ea = prof_lerp_i(dt.erow, gb + gs);
-            const float3 Eb = prof_lerp_i(dt.erow, gb - gs);
-            La = make_float3(fmaf(row.gv.x, Ea.x, La.x), fmaf(row.gv.y, Ea.y, La.y),
-                             fmaf(row.gv.z, Ea.z, La.z));
-            Lb = make_float3(fmaf(row.gv.x, Eb.x, Lb.x), fmaf(row.gv.y, Eb.y, Lb.y),
-                             fmaf(row.gv.z, Eb.z, Lb.z));
-        }
-        const float Sx = La.x + Lb.x, Sy = La.y + Lb.y, Sz = La.z + Lb.z;
-        aR.x = fmaf(q, Sx, aR.x);
-        aR.y = fmaf(q, Sy, aR.y);
-        aR.z = fmaf(q, Sz, aR.z);
-        aM.x = fmaf(qw, Sx, aM.x);
-        aM.y = fmaf(qw, Sy, aM.y);
-        aM.z = fmaf(qw, Sz, aM.z);
+        density_pair<SINGLE, GROUND>(dt, row, cx, sy, q, qw, aR, aM);
+    }
+}
+// density_row for two texels of one mu (one thread): nu2, q and q w depend
+// on mu and the direction only, so the two texels share them; each texel's
+// own operations are density_row's, in the same order (the same bits).
+template <bool SINGLE, bool GROUND>
+__device__ __forceinline__ void density_row2(const DenTexel& dA, const DenRow& rA, float3& aRA,
+                                             float3& aMA, const DenTexel& dB,
+                                             const DenRow& rB, float3& aRB, float3& aMB) {
+#pragma unroll
+    for (int m = 0; m < kDenPhi / 2; ++m) {
+        const float4 o = rA.dir[m];
+        const float cx = o.x, sy = o.y;
+        const float nu2 = fmaf(cx, dA.omx, rA.omzc);
+        const float q = fmaf(nu2, nu2, 1.0f);
+        const float qw = q * rsqrt_cubed(fmaf(dA.g2, nu2, dA.g1));
+        density_pair<SINGLE, GROUND>(dA, rA, cx, sy, q, qw, aRA, aMA);
+        density_pair<SINGLE, GROUND>(dB, rB, cx, sy, q, qw, aRB, aMB);
     }
 }
 
@@ -958,7 +991,7 @@ __device__ __forceinline__ void density_geometry_block(const DevParams& P, const
 // Per-sample body of the density quadrature for one pair of directions
 // (phi_m, phi_{31-m}): both share cos(phi), hence nu2 and the two phase
 // functions; their sin(phi) terms have opposite signs.
-template <typename G, bool SINGLE, bool H>
+template <typename G, bool SINGLE, bool H, int TPX = 1>
 __device__ __forceinline__ void scattering_density_block(const DevParams& P, const int bx, const int by, const int bz, const float4* __restrict__ DG, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J) {
@@ -1050,54 +1083,36 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
     }
     __syncthreads();
 
-    // S consecutive lanes per texel, lane l0 of a texel taking theta rows
-    // l0, l0 + S, ... (S = 1: one thread walks all 16 rows).
-    const int S = P.den_split;
-    const int t = bx * (int(blockDim.x) / S) + tid / S;  // texel within the tile
-    if (t >= MU * NU) return;  // a whole group of S lanes
-    const int l0 = tid % S;
-    const int k = t % NU, y = t / NU;
-    const unsigned x = unsigned(k * MUS + j);
-    TexelCoord<G> c = scattering_texel_coord<G>(P, x, unsigned(y), unsigned(z));
-
-    // omega, omega_s (scattering_density.comp:28-32), built in G.
-    G om_x = sqrt(G(1) - c.mu * c.mu);
-    G sdx = om_x == G(0) ? G(0) : (c.nu - c.mu * c.mu_s) / om_x;
-    G sdy = sqrt(fmax(G(1) - sdx * sdx - c.mu_s * c.mu_s, G(0)));
-    DenTexel dt;
-    dt.omx = float(om_x);
-    dt.omz = float(c.mu);
-    // Unit-vector components, clamped so every lookup coordinate is finite.
-    dt.osx = fminf(fmaxf(float(sdx), -1.0f), 1.0f);
-    dt.osy = fminf(fmaxf(float(sdy), -1.0f), 1.0f);
-    dt.osz = float(c.mu_s);
-    dt.osxh = dt.osx * (0.5f * float(NU - 1));
-    dt.osyh = dt.osy * (0.5f * float(NU - 1));
-    const float rf = float(c.r);
-    const float alt = float(c.r - G(P.bottom));
-    const float rho_r = profile_density(P.prof[0], alt);
-    const float rho_m = profile_density(P.prof[1], alt);
     const float g = P.g;
     const float kR = 3.0f / (16.0f * kPi);
     const float kM = 3.0f / (8.0f * kPi) * (1.0f - g * g) / (2.0f + g * g);
-    const float cr0 = kR * P.rayleigh_scattering[0] * rho_r,
-                cr1 = kR * P.rayleigh_scattering[1] * rho_r,
-                cr2 = kR * P.rayleigh_scattering[2] * rho_r;
-    const float cm0 = kM * P.mie_scattering[0] * rho_m, cm1 = kM * P.mie_scattering[1] * rho_m,
-                cm2 = kM * P.mie_scattering[2] * rho_m;
     const int gmask = s_ground_mask;
-    dt.g1 = 1.0f + g * g;
-    dt.g2 = -2.0f * g;
-    dt.hnu = 0.5f * float(NU - 1);
-    dt.hnu1 = dt.hnu + 1.0f;
-    dt.hE = 0.5f * float(NE - 1);
-    dt.erow = s_erow;
-
-    float3 acc = make_float3(0.0f, 0.0f, 0.0f);
-    for (int l = l0; l < kDenTheta; l += S) {
-        float3 aR = make_float3(0.0f, 0.0f, 0.0f), aM = make_float3(0.0f, 0.0f, 0.0f);
-        const float4 d0 = c_den_dir[l * kDenPhi];
-        const float ct = d0.z, dom = d0.w;
+    // omega, omega_s of texel (nu index k, mu index y) (scattering_density.
+    // comp:28-32), built in G.
+    auto texel_setup = [&](int k, int y, DenTexel& dt) {
+        const unsigned x = unsigned(k * MUS + j);
+        TexelCoord<G> c = scattering_texel_coord<G>(P, x, unsigned(y), unsigned(z));
+        G om_x = sqrt(G(1) - c.mu * c.mu);
+        G sdx = om_x == G(0) ? G(0) : (c.nu - c.mu * c.mu_s) / om_x;
+        G sdy = sqrt(fmax(G(1) - sdx * sdx - c.mu_s * c.mu_s, G(0)));
+        dt.omx = float(om_x);
+        dt.omz = float(c.mu);
+        // Unit-vector components, clamped so every lookup coordinate is finite.
+        dt.osx = fminf(fmaxf(float(sdx), -1.0f), 1.0f);
+        dt.osy = fminf(fmaxf(float(sdy), -1.0f), 1.0f);
+        dt.osz = float(c.mu_s);
+        dt.osxh = dt.osx * (0.5f * float(NU - 1));
+        dt.osyh = dt.osy * (0.5f * float(NU - 1));
+        dt.g1 = 1.0f + g * g;
+        dt.g2 = -2.0f * g;
+        dt.hnu = 0.5f * float(NU - 1);
+        dt.hnu1 = dt.hnu + 1.0f;
+        dt.hE = 0.5f * float(NE - 1);
+        dt.erow = s_erow;
+        return c;
+    };
+    // Theta row l of a texel (ct = cos(theta)).
+    auto make_row = [&](const DenTexel& dt, int l, float ct, float rf) {
         DenRow row;
         row.C = ct * dt.osz;
         row.Ch1 = fmaf(row.C, dt.hnu, dt.hnu1);
@@ -1113,26 +1128,100 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
             row.Ag = dgi * dt.osx * dt.hE;
             row.Bg = dgi * dt.osy * dt.hE;
             row.Cg1 = fmaf((rf + ct * gv.w) * inv * dt.osz, dt.hE, dt.hE + 1.0f);
-            density_row<SINGLE, true>(dt, row, aR, aM);
-        } else {
-            density_row<SINGLE, false>(dt, row, aR, aM);
         }
-        // Scattering coefficients x phase constants x solid angle.
-        acc.x = fmaf(cm0 * dom, aM.x, fmaf(cr0 * dom, aR.x, acc.x));
-        acc.y = fmaf(cm1 * dom, aM.y, fmaf(cr1 * dom, aR.y, acc.y));
-        acc.z = fmaf(cm2 * dom, aM.z, fmaf(cr2 * dom, aR.z, acc.z));
-    }
-    if (S > 1) {
-        // The S rows' sums of one texel, in S consecutive lanes of one
-        // wavefront (S divides 64): a tree reduction.
-        for (int o = S / 2; o > 0; o >>= 1) {
-            acc.x += __shfl_xor(acc.x, o);
-            acc.y += __shfl_xor(acc.y, o);
-            acc.z += __shfl_xor(acc.z, o);
+        return row;
+    };
+
+    if constexpr (TPX == 2) {
+        // Thread p of the tile: texels (mu index y, nu index k) and
+        // (y, k + NU/2), which share mu (NU even, one thread per texel pair).
+        const int half = NU / 2;
+        const int p = bx * int(blockDim.x) + tid;
+        if (p >= MU * half) return;
+        const int kA = p % half, y = p / half, kB = kA + half;
+        DenTexel dA, dB;
+        const TexelCoord<G> cA = texel_setup(kA, y, dA);
+        texel_setup(kB, y, dB);
+        const float rf = float(cA.r);
+        const float alt = float(cA.r - G(P.bottom));
+        const float rho_r = profile_density(P.prof[0], alt);
+        const float rho_m = profile_density(P.prof[1], alt);
+        const float cr0 = kR * P.rayleigh_scattering[0] * rho_r,
+                    cr1 = kR * P.rayleigh_scattering[1] * rho_r,
+                    cr2 = kR * P.rayleigh_scattering[2] * rho_r;
+        const float cm0 = kM * P.mie_scattering[0] * rho_m,
+                    cm1 = kM * P.mie_scattering[1] * rho_m,
+                    cm2 = kM * P.mie_scattering[2] * rho_m;
+        float3 accA = make_float3(0.0f, 0.0f, 0.0f), accB = make_float3(0.0f, 0.0f, 0.0f);
+        for (int l = 0; l < kDenTheta; ++l) {
+            float3 aRA = make_float3(0.0f, 0.0f, 0.0f), aMA = make_float3(0.0f, 0.0f, 0.0f);
+            float3 aRB = make_float3(0.0f, 0.0f, 0.0f), aMB = make_float3(0.0f, 0.0f, 0.0f);
+            const float4 d0 = c_den_dir[l * kDenPhi];
+            const float ct = d0.z, dom = d0.w;
+            const DenRow rA = make_row(dA, l, ct, rf), rB = make_row(dB, l, ct, rf);
+            if ((gmask >> l) & 1)
+                density_row2<SINGLE, true>(dA, rA, aRA, aMA, dB, rB, aRB, aMB);
+            else
+                density_row2<SINGLE, false>(dA, rA, aRA, aMA, dB, rB, aRB, aMB);
+            accA.x = fmaf(cm0 * dom, aMA.x, fmaf(cr0 * dom, aRA.x, accA.x));
+            accA.y = fmaf(cm1 * dom, aMA.y, fmaf(cr1 * dom, aRA.y, accA.y));
+            accA.z = fmaf(cm2 * dom, aMA.z, fmaf(cr2 * dom, aRA.z, accA.z));
+            accB.x = fmaf(cm0 * dom, aMB.x, fmaf(cr0 * dom, aRB.x, accB.x));
+            accB.y = fmaf(cm1 * dom, aMB.y, fmaf(cr1 * dom, aRB.y, accB.y));
+            accB.z = fmaf(cm2 * dom, aMB.z, fmaf(cr2 * dom, aRB.z, accB.z));
         }
-        if (l0 != 0) return;
+        const size_t rowbase = (size_t(z) * MU + y) * W;
+        st_tex(J, rowbase + size_t(kA * MUS + j), make_float4(accA.x, accA.y, accA.z, 0.0f), H);
+        st_tex(J, rowbase + size_t(kB * MUS + j), make_float4(accB.x, accB.y, accB.z, 0.0f), H);
+        return;
+    } else {
+        // S consecutive lanes per texel, lane l0 of a texel taking theta rows
+        // l0, l0 + S, ... (S = 1: one thread walks all 16 rows).
+        const int S = P.den_split;
+        const int t = bx * (int(blockDim.x) / S) + tid / S;  // texel within the tile
+        if (t >= MU * NU) return;  // a whole group of S lanes
+        const int l0 = tid % S;
+        const int k = t % NU, y = t / NU;
+        const unsigned x = unsigned(k * MUS + j);
+        DenTexel dt;
+        const TexelCoord<G> c = texel_setup(k, y, dt);
+        const float rf = float(c.r);
+        const float alt = float(c.r - G(P.bottom));
+        const float rho_r = profile_density(P.prof[0], alt);
+        const float rho_m = profile_density(P.prof[1], alt);
+        const float cr0 = kR * P.rayleigh_scattering[0] * rho_r,
+                    cr1 = kR * P.rayleigh_scattering[1] * rho_r,
+                    cr2 = kR * P.rayleigh_scattering[2] * rho_r;
+        const float cm0 = kM * P.mie_scattering[0] * rho_m,
+                    cm1 = kM * P.mie_scattering[1] * rho_m,
+                    cm2 = kM * P.mie_scattering[2] * rho_m;
+        float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+        for (int l = l0; l < kDenTheta; l += S) {
+            float3 aR = make_float3(0.0f, 0.0f, 0.0f), aM = make_float3(0.0f, 0.0f, 0.0f);
+            const float4 d0 = c_den_dir[l * kDenPhi];
+            const float ct = d0.z, dom = d0.w;
+            const DenRow row = make_row(dt, l, ct, rf);
+            if ((gmask >> l) & 1)
+                density_row<SINGLE, true>(dt, row, aR, aM);
+            else
+                density_row<SINGLE, false>(dt, row, aR, aM);
+            // Scattering coefficients x phase constants x solid angle.
+            acc.x = fmaf(cm0 * dom, aM.x, fmaf(cr0 * dom, aR.x, acc.x));
+            acc.y = fmaf(cm1 * dom, aM.y, fmaf(cr1 * dom, aR.y, acc.y));
+            acc.z = fmaf(cm2 * dom, aM.z, fmaf(cr2 * dom, aR.z, acc.z));
+        }
+        if (S > 1) {
+            // The S rows' sums of one texel, in S consecutive lanes of one
+            // wavefront (S divides 64): a tree reduction.
+            for (int o = S / 2; o > 0; o >>= 1) {
+                acc.x += __shfl_xor(acc.x, o);
+                acc.y += __shfl_xor(acc.y, o);
+                acc.z += __shfl_xor(acc.z, o);
+            }
+            if (l0 != 0) return;
+        }
+        st_tex(J, (size_t(z) * MU + y) * W + x, make_float4(acc.x, acc.y, acc.z, 0.0f), H);
     }
-    st_tex(J, (size_t(z) * MU + y) * W + x, make_float4(acc.x, acc.y, acc.z, 0.0f), H);
 }
 
 
@@ -1783,14 +1872,14 @@ __global__ __launch_bounds__(kSsChunkThreads) void kb_single_scattering_chunked(
     }
 }
 
-template <typename G, bool SINGLE, bool H>
-__global__ __launch_bounds__(256, 8) void k_scattering_density(
+template <typename G, bool SINGLE, bool H, int TPX = 1>
+__global__ __launch_bounds__(256, TPX == 2 ? kDenPairOccupancy : 8) void k_scattering_density(
     DevParams P, const float4* __restrict__ DG, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J, unsigned remap) {
     const unsigned b = logical_block(remap), gx = gridDim.x;
-    scattering_density_block<G, SINGLE, H>(P, b % gx, (b / gx) % P.s_mus, b / (gx * P.s_mus),
-                                           DG, dR, dM, dMS, dE, J);
+    scattering_density_block<G, SINGLE, H, TPX>(P, b % gx, (b / gx) % P.s_mus,
+                                                b / (gx * P.s_mus), DG, dR, dM, dMS, dE, J);
 }
 
 // Per-set geometry: blocks [0, R + MUS) fill the density geometry table,
@@ -1828,10 +1917,10 @@ __global__ __launch_bounds__(64) void kb_geometry(const DevParams* __restrict__ 
         geometry_block<G>(Ps[atm], local, l.T, l.OD, l.DG, l.SR, l.MR);
     }
 }
-template <typename G, bool SINGLE, bool H>
-__global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams* __restrict__ Ps,
-                                                                const LutPtrs* __restrict__ L,
-                                                                const uint32_t* __restrict__ tiles, unsigned ntiles) {
+template <typename G, bool SINGLE, bool H, int TPX = 1>
+__global__ __launch_bounds__(256, TPX == 2 ? kDenPairOccupancy : 8) void kb_scattering_density(
+    const DevParams* __restrict__ Ps, const LutPtrs* __restrict__ L,
+    const uint32_t* __restrict__ tiles, unsigned ntiles) {
     // Persistent when the grid is smaller than the tile list: the block
     // takes tiles blockIdx.x, +gridDim.x, ...; the barrier keeps one
     // tile's LDS reads apart from the next tile's writes.
@@ -1841,11 +1930,11 @@ __global__ __launch_bounds__(256, 8) void kb_scattering_density(const DevParams*
         decode_tile(tiles[t], atm, local);
         const DevParams& P = Ps[atm];
         const LutPtrs l = lut_ptrs(L, atm);
-        const int per = int(blockDim.x) / P.den_split;  // texels per block
-        const int gx = (P.s_mu * P.s_nu + per - 1) / per;
-        scattering_density_block<G, SINGLE, H>(P, local % gx, (local / gx) % P.s_mus,
-                                               local / (gx * P.s_mus), l.DG, l.dR, l.dM, l.dMS,
-                                               l.dE, l.J);
+        const int per = int(blockDim.x) / P.den_split;  // texels (pairs at TPX 2) per block
+        const int gx = (P.s_mu * P.s_nu / TPX + per - 1) / per;
+        scattering_density_block<G, SINGLE, H, TPX>(P, local % gx, (local / gx) % P.s_mus,
+                                                    local / (gx * P.s_mus), l.DG, l.dR, l.dM,
+                                                    l.dMS, l.dE, l.J);
     }
 }
 
@@ -2028,6 +2117,18 @@ static hipError_t launch_scattering_density_h(const DevParams& P, int order, con
     const bool single = (order - 1) == 1;
     size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
     const bool precise = P.flags & 1;
+    if (den_tpx(P) == 2) {
+        grid.x = cdiv(size_t(P.s_mu) * (P.s_nu / 2), threads);
+        if (precise && single)
+            k_scattering_density<double, true, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        else if (precise)
+            k_scattering_density<double, false, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        else if (single)
+            k_scattering_density<float, true, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        else
+            k_scattering_density<float, false, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+        return hipGetLastError();
+    }
     if (precise && single)
         k_scattering_density<double, true, H><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
     else if (precise)
@@ -2192,7 +2293,8 @@ PassShape batch_pass_shape(const DevParams& P, int pass, int order, unsigned thr
         case kPassGeometry:
             return {unsigned(P.s_r + P.s_mus + P.s_mu * P.s_r), 64, 0};
         case kPassDensity:
-            return {cdiv(size_t(P.s_mu) * P.s_nu, threads / unsigned(P.den_split)) *
+            return {cdiv(size_t(P.s_mu) * P.s_nu / unsigned(den_tpx(P)),
+                         threads / unsigned(P.den_split)) *
                         unsigned(P.s_mus) * unsigned(P.s_r),
                     threads, density_lds_bytes(P.s_nu, P.i_mus, (order - 1) == 1)};
         case kPassIndirect: {  // one wave per texel
@@ -2256,15 +2358,20 @@ static hipError_t launch_batch_h(int pass, int order, bool precise, const DevPar
             else
                 kb_geometry<float><<<grid, block, 0, s>>>(Ps, L, tiles, ntiles);
             break;
-        case kPassDensity:
-            if (precise && single)
-                kb_scattering_density<double, true, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
-            else if (precise)
-                kb_scattering_density<double, false, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
-            else if (single)
-                kb_scattering_density<float, true, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
-            else
-                kb_scattering_density<float, false, H><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles);
+        case kPassDensity:  // tpt: texels per thread (den_tpx)
+#define FB_KBD(G, SG, X) kb_scattering_density<G, SG, H, X><<<grid, block, lds, s>>>(Ps, L, tiles, ntiles)
+            if (tpt == 2) {
+                if (precise && single) FB_KBD(double, true, 2);
+                else if (precise) FB_KBD(double, false, 2);
+                else if (single) FB_KBD(float, true, 2);
+                else FB_KBD(float, false, 2);
+            } else {
+                if (precise && single) FB_KBD(double, true, 1);
+                else if (precise) FB_KBD(double, false, 1);
+                else if (single) FB_KBD(float, true, 1);
+                else FB_KBD(float, false, 1);
+            }
+#undef FB_KBD
             break;
         case kPassIndirect:
             if (precise && single)
@@ -2338,10 +2445,16 @@ static unsigned batch_resident_blocks_h(int pass, int order, bool precise, unsig
             fn = precise ? (const void*)kb_geometry<double> : (const void*)kb_geometry<float>;
             break;
         case kPassDensity:
-            fn = precise ? (single ? (const void*)kb_scattering_density<double, true, H>
-                                   : (const void*)kb_scattering_density<double, false, H>)
-                         : (single ? (const void*)kb_scattering_density<float, true, H>
-                                   : (const void*)kb_scattering_density<float, false, H>);
+            if (tpt == 2)
+                fn = precise ? (single ? (const void*)kb_scattering_density<double, true, H, 2>
+                                       : (const void*)kb_scattering_density<double, false, H, 2>)
+                             : (single ? (const void*)kb_scattering_density<float, true, H, 2>
+                                       : (const void*)kb_scattering_density<float, false, H, 2>);
+            else
+                fn = precise ? (single ? (const void*)kb_scattering_density<double, true, H, 1>
+                                       : (const void*)kb_scattering_density<double, false, H, 1>)
+                             : (single ? (const void*)kb_scattering_density<float, true, H, 1>
+                                       : (const void*)kb_scattering_density<float, false, H, 1>);
             break;
         case kPassIndirect:
             fn = precise ? (single ? (const void*)kb_indirect_irradiance<double, true, H>

@@ -60,6 +60,30 @@ def test_batch_members_equal_single_builds(builder, precise):
         pa.close()
 
 
+@pytest.mark.parametrize("precise", [False, True], ids=["fp32", "precise"])
+def test_batch_default_size_members_equal_single_builds(builder, precise):
+    """Default-size members (density with two texels per thread, beside
+    half-resolution members with one) equal their plain builds bit for
+    bit."""
+    rng = np.random.default_rng(22)
+    fl = abi.FB_FLAG_PRECISE if precise else 0
+    ps = [Parameters().replace(order=3, flags=fl),
+          random_atmosphere(rng, precise, resolution="default", order=4),
+          random_atmosphere(rng, precise, resolution="half", order=3)]
+    print(f"\n[batch] default-size members, precise={precise}", flush=True)
+    pend = Atmosphere.build_batch(builder, None, ps)
+    for pa in pend:
+        pa.wait()
+    for p, pb in zip(ps, pend):
+        pa = Atmosphere.build(builder, None, p)
+        pa.wait()
+        for k in KEYS:
+            assert np.array_equal(pa.atmosphere().download(k), pb.atmosphere().download(k)), k
+        pa.close()
+    for pa in pend:
+        pa.close()
+
+
 def test_batch_member_vs_fp64_oracle(builder):
     ps = small_mix(True)
     pend = Atmosphere.build_batch(builder, None, ps)
