@@ -108,10 +108,13 @@ constexpr size_t kDenSplitTexels = 65536;
 // nu2 = omega.omega_i, q and the Cornette-Shanks power of every direction
 // pair (computed once for both; each texel's own operations and their order
 // are unchanged, so the values are the same bits), and a block's profile
-// build serves twice the texels.  At most kDenPairOccupancy blocks of 256
-// threads per CU then (the pair's registers).
+// build serves twice the texels.
 constexpr int kDenPairMinTexels = 512;
-constexpr int kDenPairOccupancy = 5;
+// Blocks per CU the density kernels are compiled for: 8 at one texel per
+// thread (64 VGPRs), 5 at two (91-96).  Four texels per thread (3 blocks,
+// 138-143 VGPRs) measured 6% slower at double and 30% at default sizes; at
+// two, 4 blocks for order 2 (no spills) was 1% slower than 5.
+__host__ __device__ constexpr int den_occupancy(int tpx, bool) { return tpx == 1 ? 8 : 5; }
 __host__ __device__ inline int den_tpx(const DevParams& P) {
     return (P.den_split == 1 && P.s_nu % 2 == 0 && P.s_mu * P.s_nu >= kDenPairMinTexels) ? 2
                                                                                           : 1;

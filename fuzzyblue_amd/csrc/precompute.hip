@@ -898,22 +898,22 @@ __device__ __forceinline__ void density_row(const DenTexel& dt, const DenRow& ro
         density_pair<SINGLE, GROUND>(dt, row, cx, sy, q, qw, aR, aM);
     }
 }
-// density_row for two texels of one mu (one thread): nu2, q and q w depend
-// on mu and the direction only, so the two texels share them; each texel's
-// own operations are density_row's, in the same order (the same bits).
-template <bool SINGLE, bool GROUND>
-__device__ __forceinline__ void density_row2(const DenTexel& dA, const DenRow& rA, float3& aRA,
-                                             float3& aMA, const DenTexel& dB,
-                                             const DenRow& rB, float3& aRB, float3& aMB) {
+// density_row for N texels of one mu (one thread): nu2, q and q w depend
+// on mu and the direction only, so the texels share them; each texel's own
+// operations are density_row's, in the same order (the same bits).
+template <bool SINGLE, bool GROUND, int N>
+__device__ __forceinline__ void density_row_n(const DenTexel* dt, const DenRow* row, float3* aR,
+                                              float3* aM) {
 #pragma unroll
     for (int m = 0; m < kDenPhi / 2; ++m) {
-        const float4 o = rA.dir[m];
+        const float4 o = row[0].dir[m];
         const float cx = o.x, sy = o.y;
-        const float nu2 = fmaf(cx, dA.omx, rA.omzc);
+        const float nu2 = fmaf(cx, dt[0].omx, row[0].omzc);
         const float q = fmaf(nu2, nu2, 1.0f);
-        const float qw = q * rsqrt_cubed(fmaf(dA.g2, nu2, dA.g1));
-        density_pair<SINGLE, GROUND>(dA, rA, cx, sy, q, qw, aRA, aMA);
-        density_pair<SINGLE, GROUND>(dB, rB, cx, sy, q, qw, aRB, aMB);
+        const float qw = q * rsqrt_cubed(fmaf(dt[0].g2, nu2, dt[0].g1));
+#pragma unroll
+        for (int u = 0; u < N; ++u)
+            density_pair<SINGLE, GROUND>(dt[u], row[u], cx, sy, q, qw, aR[u], aM[u]);
     }
 }
 
@@ -1132,18 +1132,20 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         return row;
     };
 
-    if constexpr (TPX == 2) {
-        // Thread p of the tile: texels (mu index y, nu index k) and
-        // (y, k + NU/2), which share mu (NU even, one thread per texel pair).
-        const int half = NU / 2;
+    if constexpr (TPX > 1) {
+        // Thread p of the tile: texels (mu index y, nu index k + u NU/TPX),
+        // u < TPX, which share mu (NU a multiple of TPX; one thread per
+        // group of TPX texels).
+        const int part = NU / TPX;
         const int p = bx * int(blockDim.x) + tid;
-        if (p >= MU * half) return;
-        const int kA = p % half, y = p / half, kB = kA + half;
-        DenTexel dA, dB;
-        const TexelCoord<G> cA = texel_setup(kA, y, dA);
-        texel_setup(kB, y, dB);
-        const float rf = float(cA.r);
-        const float alt = float(cA.r - G(P.bottom));
+        if (p >= MU * part) return;
+        const int k0 = p % part, y = p / part;
+        DenTexel dts[TPX];
+        TexelCoord<G> c0 = texel_setup(k0, y, dts[0]);
+#pragma unroll
+        for (int u = 1; u < TPX; ++u) texel_setup(k0 + u * part, y, dts[u]);
+        const float rf = float(c0.r);
+        const float alt = float(c0.r - G(P.bottom));
         const float rho_r = profile_density(P.prof[0], alt);
         const float rho_m = profile_density(P.prof[1], alt);
         const float cr0 = kR * P.rayleigh_scattering[0] * rho_r,
@@ -1152,27 +1154,36 @@ __device__ __forceinline__ void scattering_density_block(const DevParams& P, con
         const float cm0 = kM * P.mie_scattering[0] * rho_m,
                     cm1 = kM * P.mie_scattering[1] * rho_m,
                     cm2 = kM * P.mie_scattering[2] * rho_m;
-        float3 accA = make_float3(0.0f, 0.0f, 0.0f), accB = make_float3(0.0f, 0.0f, 0.0f);
+        float3 acc[TPX];
+#pragma unroll
+        for (int u = 0; u < TPX; ++u) acc[u] = make_float3(0.0f, 0.0f, 0.0f);
         for (int l = 0; l < kDenTheta; ++l) {
-            float3 aRA = make_float3(0.0f, 0.0f, 0.0f), aMA = make_float3(0.0f, 0.0f, 0.0f);
-            float3 aRB = make_float3(0.0f, 0.0f, 0.0f), aMB = make_float3(0.0f, 0.0f, 0.0f);
+            float3 aR[TPX], aM[TPX];
+            DenRow rows[TPX];
             const float4 d0 = c_den_dir[l * kDenPhi];
             const float ct = d0.z, dom = d0.w;
-            const DenRow rA = make_row(dA, l, ct, rf), rB = make_row(dB, l, ct, rf);
+#pragma unroll
+            for (int u = 0; u < TPX; ++u) {
+                aR[u] = make_float3(0.0f, 0.0f, 0.0f);
+                aM[u] = make_float3(0.0f, 0.0f, 0.0f);
+                rows[u] = make_row(dts[u], l, ct, rf);
+            }
             if ((gmask >> l) & 1)
-                density_row2<SINGLE, true>(dA, rA, aRA, aMA, dB, rB, aRB, aMB);
+                density_row_n<SINGLE, true, TPX>(dts, rows, aR, aM);
             else
-                density_row2<SINGLE, false>(dA, rA, aRA, aMA, dB, rB, aRB, aMB);
-            accA.x = fmaf(cm0 * dom, aMA.x, fmaf(cr0 * dom, aRA.x, accA.x));
-            accA.y = fmaf(cm1 * dom, aMA.y, fmaf(cr1 * dom, aRA.y, accA.y));
-            accA.z = fmaf(cm2 * dom, aMA.z, fmaf(cr2 * dom, aRA.z, accA.z));
-            accB.x = fmaf(cm0 * dom, aMB.x, fmaf(cr0 * dom, aRB.x, accB.x));
-            accB.y = fmaf(cm1 * dom, aMB.y, fmaf(cr1 * dom, aRB.y, accB.y));
-            accB.z = fmaf(cm2 * dom, aMB.z, fmaf(cr2 * dom, aRB.z, accB.z));
+                density_row_n<SINGLE, false, TPX>(dts, rows, aR, aM);
+#pragma unroll
+            for (int u = 0; u < TPX; ++u) {
+                acc[u].x = fmaf(cm0 * dom, aM[u].x, fmaf(cr0 * dom, aR[u].x, acc[u].x));
+                acc[u].y = fmaf(cm1 * dom, aM[u].y, fmaf(cr1 * dom, aR[u].y, acc[u].y));
+                acc[u].z = fmaf(cm2 * dom, aM[u].z, fmaf(cr2 * dom, aR[u].z, acc[u].z));
+            }
         }
         const size_t rowbase = (size_t(z) * MU + y) * W;
-        st_tex(J, rowbase + size_t(kA * MUS + j), make_float4(accA.x, accA.y, accA.z, 0.0f), H);
-        st_tex(J, rowbase + size_t(kB * MUS + j), make_float4(accB.x, accB.y, accB.z, 0.0f), H);
+#pragma unroll
+        for (int u = 0; u < TPX; ++u)
+            st_tex(J, rowbase + size_t((k0 + u * part) * MUS + j),
+                   make_float4(acc[u].x, acc[u].y, acc[u].z, 0.0f), H);
         return;
     } else {
         // S consecutive lanes per texel, lane l0 of a texel taking theta rows
@@ -1873,7 +1884,7 @@ __global__ __launch_bounds__(kSsChunkThreads) void kb_single_scattering_chunked(
 }
 
 template <typename G, bool SINGLE, bool H, int TPX = 1>
-__global__ __launch_bounds__(256, TPX == 2 ? kDenPairOccupancy : 8) void k_scattering_density(
+__global__ __launch_bounds__(256, den_occupancy(TPX, SINGLE)) void k_scattering_density(
     DevParams P, const float4* __restrict__ DG, const float4* __restrict__ dR,
     const float4* __restrict__ dM, const float4* __restrict__ dMS, const float4* __restrict__ dE,
     float4* __restrict__ J, unsigned remap) {
@@ -1918,7 +1929,7 @@ __global__ __launch_bounds__(64) void kb_geometry(const DevParams* __restrict__ 
     }
 }
 template <typename G, bool SINGLE, bool H, int TPX = 1>
-__global__ __launch_bounds__(256, TPX == 2 ? kDenPairOccupancy : 8) void kb_scattering_density(
+__global__ __launch_bounds__(256, den_occupancy(TPX, SINGLE)) void kb_scattering_density(
     const DevParams* __restrict__ Ps, const LutPtrs* __restrict__ L,
     const uint32_t* __restrict__ tiles, unsigned ntiles) {
     // Persistent when the grid is smaller than the tile list: the block
@@ -2117,16 +2128,16 @@ static hipError_t launch_scattering_density_h(const DevParams& P, int order, con
     const bool single = (order - 1) == 1;
     size_t lds = density_lds_bytes(P.s_nu, P.i_mus, single);
     const bool precise = P.flags & 1;
-    if (den_tpx(P) == 2) {
-        grid.x = cdiv(size_t(P.s_mu) * (P.s_nu / 2), threads);
-        if (precise && single)
-            k_scattering_density<double, true, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
-        else if (precise)
-            k_scattering_density<double, false, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
-        else if (single)
-            k_scattering_density<float, true, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
-        else
-            k_scattering_density<float, false, H, 2><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap);
+    const int tpx = den_tpx(P);
+    if (tpx > 1) {
+        grid.x = cdiv(size_t(P.s_mu) * (P.s_nu / tpx), threads);
+#define FB_KD(G, SG, X) \
+    k_scattering_density<G, SG, H, X><<<grid, threads, lds, s>>>(P, DG, dR, dM, dMS, dE, J, remap)
+        if (precise && single) FB_KD(double, true, 2);
+        else if (precise) FB_KD(double, false, 2);
+        else if (single) FB_KD(float, true, 2);
+        else FB_KD(float, false, 2);
+#undef FB_KD
         return hipGetLastError();
     }
     if (precise && single)
