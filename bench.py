@@ -476,7 +476,7 @@ def measure_density_roofline(b, stream, ceilings, steps=PROFILE_STEPS):
     peak = max(v for k, v in ceilings.items() if "b128_b64_tbps" in k)
     return {
         "bound": "lds",
-        "kernel": "k_scattering_density<double,*,false>",
+        "kernel": "k_scattering_density<double,*,false,2>",
         "workload": "Earth, scattering 64x256x64x16, order 4, precision mode",
         "achieved": achieved,
         "peak": peak,
