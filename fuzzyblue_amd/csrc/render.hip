@@ -645,9 +645,13 @@ static fb_status begin_use(fb_atmosphere* a, hipStream_t s) {
     FB_HIP_TRY(hipStreamWaitEvent(s, a->ready, 0));
     return FB_OK;
 }
-static fb_status end_use(fb_atmosphere* a, hipStream_t s) {
-    FB_HIP_TRY(hipGetLastError());
-    return atmosphere_mark_use(a, s);
+// The use is recorded even when a launch failed: work queued before the
+// failing launch may still read the LUTs.
+static fb_status end_use(fb_atmosphere* a, hipStream_t s, hipError_t e = hipSuccess) {
+    if (e == hipSuccess) e = hipGetLastError();
+    const fb_status st = atmosphere_mark_use(a, s);
+    if (e != hipSuccess) return hip_fail(e, "render launch");
+    return st;
 }
 
 extern "C" {
@@ -770,15 +774,12 @@ fb_status fb_render_sky_frames(fb_builder* b, fb_stream stream, const fb_atmosph
     grid.x = (grid.x + fbk::kPerRayStride - 1) / fbk::kPerRayStride;
     h ? FB_FRAMES(true, false) : FB_FRAMES(false, false);
 #undef FB_FRAMES
+    hipError_t e = hipGetLastError();
     if (slices) {
-        const hipError_t le = hipGetLastError();
         const hipError_t fe = hipFreeAsync(slices, s);
-        if (le != hipSuccess || fe != hipSuccess) {
-            set_error("frame launch: %s", hipGetErrorString(le != hipSuccess ? le : fe));
-            return FB_EHIP;
-        }
+        if (e == hipSuccess) e = fe;
     }
-    return end_use(a, s);
+    return end_use(a, s, e);
 }
 
 void fb_ray_direction(uint64_t seed, uint32_t frame, uint32_t pixel, float out[3]) {
