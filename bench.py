@@ -259,7 +259,7 @@ def measure_batch(b, stream, ps, label):
         "speedup_vs_plain": ms_plain / ms_batch,
         "batched_ms": ms_batch,
         "plain_ms": ms_plain,
-        "survey_roofline_ms": work * 1e3,
+        "costmodel_roofline_ms": work * 1e3,
     }
     log(f"[bench] {label}: {out}")
     return out

@@ -478,7 +478,7 @@ fb_status fb_render_sky_frames(fb_builder *b, fb_stream stream,
                                float *transmittance);
 
 /* The direction generator fb_render_sky_frames uses, on the host (so a
- * checker can rebuild the same rays). */
+ * test can rebuild the same rays). */
 void fb_ray_direction(uint64_t seed, uint32_t frame, uint32_t pixel,
                       float out[3]);
 

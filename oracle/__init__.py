@@ -1,8 +1,8 @@
 """ctypes loader for the CPU oracle (oracle/libfb_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker.  The product never
-imports this package.
+cpu_baseline leg of bench.py, always as the reference the GPU results are
+compared with.  The product never imports this package.
 
 Two precisions are exposed with the same API: ``Oracle("f64")`` — the FP64
 oracle (the reference's discretisation evaluated in double) — and

@@ -2,9 +2,10 @@
  * atmo_oracle.c — CPU restatement of fuzzyblue's precompute passes and sky /
  * lighting lookups.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
- * product (fuzzyblue_amd/, libfuzzyblue_hip.so) never links or calls it.
+ * TEST INFRASTRUCTURE ONLY: the CPU reference the GPU results are compared
+ * with.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  The product (fuzzyblue_amd/, libfuzzyblue_hip.so) never
+ * links or calls it.
  *
  * It is compiled twice from this one source:
  *   ORACLE_REAL=double  -> *_f64 symbols: the FP64 oracle.  It evaluates the

@@ -1,6 +1,6 @@
 """The serving loop (fb_server_*): continuous batching of precompute and
-render requests, HBM-budget admission with LRU eviction, and the
-north-star check that each result is the same alone and in the mix."""
+render requests, HBM-budget admission with LRU eviction, and the check
+that each result is the same built alone and served in a mix."""
 import numpy as np
 import pytest
 
