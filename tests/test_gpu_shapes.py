@@ -6,12 +6,27 @@
     would leave such texels unwritten, src/precompute.rs:1742-1744; the
     build uses ceil-div grids);
   * widenu: the largest nu extent the kernels accept (32 slices);
-  * widerow: the widest scattering row (NU x MUS = 1024 texels).
+  * widerow: the widest scattering row (NU x MUS = 1024 texels);
+  * pairs: enough texels (67,392) for one thread per texel in the density
+    pass, with NU even (18, not a power of two), so each thread takes two
+    texels (mu, nu) and (mu, nu + 9); an (r, mu_s) tile's 324 threads end
+    in a partial block.
 
 FP32 mode against the FP32 restatement and precision mode against the FP64
 oracle, both within 1e-3 (floor 1e-6 x max; transmittance 1e-5), and the
 binary16 storage mode against the restatement in binary16 mode within 3e-3
 (floor 1e-3 x max).
+
+Except for "pairs" in FP32 and binary16 modes, which are held to an error
+bound relative to the LUT's maximum (1e-5 x max FP32, 1e-3 x max binary16;
+measured 3.2e-6 and 5.2e-4).  With 18 nu slices of 26 mu_s texels the
+binary32 texture coordinate ((slice + u_mu_s) / NU) * W - 0.5
+(scattering.h:149-154) of a lookup at mu_s_min rounds up to a few 1e-5 of
+a texel into the previous nu slice, whose last texel (mu_s_max) is up to
+1e7 times brighter: the darkest texels then differ between any two binary32
+evaluations that round differently.  The FP32 restatement itself is 2.4e-2
+x max from the FP64 oracle at this shape, and precision mode, which
+computes the coordinates in FP64, meets the relative bar.
 """
 import numpy as np
 import pytest
@@ -29,7 +44,14 @@ SHAPES = {
                    scattering_nu_size=32),
     "widerow": dict(scattering_r_size=3, scattering_mu_size=8, scattering_mu_s_size=64,
                     scattering_nu_size=16),
+    "pairs": dict(scattering_r_size=4, scattering_mu_size=36, scattering_mu_s_size=26,
+                  scattering_nu_size=18),
 }
+
+
+# Bounds relative to the LUT's maximum for (shape, mode) pairs whose binary32
+# texture coordinates leak across nu slices (see the module docstring).
+MAX_BOUND = {("pairs", "fp32"): 1e-5, ("pairs", "f16"): 1e-3}
 
 
 def rel_err(x, y, floor):
@@ -57,6 +79,9 @@ def test_shape_vs_oracle(builder, name, mode):
     ref = Oracle("f64" if mode == "precise" else "f32").precompute(p, fp16=mode == "f16")
     floor, tol = (1e-3, 3e-3) if mode == "f16" else (1e-6, 1e-3)
     eT = rel_err(got["transmittance"], ref["T"], 1e-6).max()
+    if (name, mode) in MAX_BOUND:
+        # floor 1: the error relative to the maximum
+        floor, tol = 1.0, MAX_BOUND[(name, mode)]
     eE = rel_err(got["irradiance"], ref["E"], floor).max()
     eS = rel_err(got["scattering"], ref["S"], floor).max()
     print(f"  T {eT:.2e} E {eE:.2e} S {eS:.2e}", flush=True)
