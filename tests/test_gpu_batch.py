@@ -1,6 +1,6 @@
 """Batched precompute (fb_atmosphere_build_batch): every atmosphere of a
 mixed batch — uneven LUT sizes, orders and parameters in one launch per
-pass — is bit-identical to the same atmosphere built alone (the batched
+pass and kernel variant — is bit-identical to the same atmosphere built alone (the batched
 kernels run the same per-block code on the same inputs), the captured graph
 replays to the same bits, and a batch member agrees with the oracle."""
 import numpy as np
@@ -28,6 +28,15 @@ def small_mix(precise):
         # rows of 32 texels: chunked multiple scattering, as in a plain build
         Parameters().replace(scattering_r_size=8, scattering_mu_size=32,
                              scattering_mu_s_size=8, scattering_nu_size=4, order=4),
+        # two density texels per thread, a tile's 324 threads ending in a
+        # partial block
+        Parameters().replace(scattering_r_size=4, scattering_mu_size=36,
+                             scattering_mu_s_size=26, scattering_nu_size=18, order=3),
+        # extents that fill no grid or tile evenly
+        Parameters().replace(transmittance_mu_size=7, transmittance_r_size=5,
+                             scattering_r_size=5, scattering_mu_size=6, scattering_mu_s_size=3,
+                             scattering_nu_size=3, irradiance_mu_s_size=5, irradiance_r_size=3,
+                             order=3),
     ]
     for p in ps:
         p.flags = abi.FB_FLAG_PRECISE if precise else 0
