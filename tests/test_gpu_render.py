@@ -192,6 +192,52 @@ def test_frames_slice_and_general_paths(builder, atm):
         assert rel_err(gt[f], o_tr, 1e-6).max() < 1e-4, f
 
 
+def test_frames_and_rays_odd_extents(builder):
+    """LUT extents that are no powers of two (transmittance 250 x 60,
+    scattering 4 x 36 x 26 x 18, irradiance 60 x 14): the frames call's
+    per-frame LUT slice (19 nu slices x 36 mu entries and the transmittance
+    row) and the per-ray kernel both give the FP32 oracle's radiance and
+    transmittance on the same hashed rays (1e-4, floor 1e-6 x max)."""
+    p = Parameters().replace(transmittance_mu_size=250, transmittance_r_size=60,
+                             scattering_r_size=4, scattering_mu_size=36,
+                             scattering_mu_s_size=26, scattering_nu_size=18,
+                             irradiance_mu_s_size=60, irradiance_r_size=14)
+    pa = Atmosphere.build(builder, None, p)
+    pa.wait()
+    a = pa.assert_ready()
+    T = a.download("transmittance").astype(np.float64)
+    S = a.download("scattering").astype(np.float64)
+    frames, px = 3, 6000
+    alt = np.array([0.2, 3.0, 9.0], np.float32)
+    zen = np.radians(np.array([20.0, 88.0, 100.0])).astype(np.float32)
+    r = Renderer(builder)
+    d_alt, d_zen = DeviceBuffer.from_array(alt), DeviceBuffer.from_array(zen)
+    rad, tr = DeviceBuffer(frames * px * 16), DeviceBuffer(frames * px * 16)
+    print(f"\n[render] odd extents {p.scattering_extent()}: {frames} frames x {px}",
+          flush=True)
+    r.frames(None, a, d_alt, d_zen, frames, px, 9, rad, tr)
+    synchronize()
+    g = rad.to_array(np.float32, (frames, px, 4))
+    gt = tr.to_array(np.float32, (frames, px, 4))
+    for f in range(frames):
+        views = np.stack([ray_direction(9, f, i) for i in range(px)]).astype(np.float32)
+        cams = np.array([[0.0, 0.0, np.float32(p.bottom_radius) + alt[f]]], np.float32)
+        sun = np.array([[math.sin(zen[f]), 0.0, math.cos(zen[f])]], np.float32)
+        d_c, d_v, d_s = (DeviceBuffer.from_array(x) for x in (cams, views, sun))
+        rr, rt = DeviceBuffer(px * 16), DeviceBuffer(px * 16)
+        r.rays(None, a, d_c, 0, d_v, d_s, 0, px, rr, rt)
+        synchronize()
+        g_rr = rr.to_array(np.float32, (px, 4))
+        g_rt = rt.to_array(np.float32, (px, 4))
+        o_rad, o_tr = Oracle("f32").sky_rays(p, T, S, cams, views, sun)
+        errs = [rel_err(x, y, 1e-6).max()
+                for x, y in ((g[f], o_rad), (gt[f], o_tr), (g_rr, o_rad), (g_rt, o_tr))]
+        print(f"  frame {f}: frames {errs[0]:.2e}/{errs[1]:.2e} rays {errs[2]:.2e}/{errs[3]:.2e}",
+              flush=True)
+        assert max(errs) < 1e-4, (f, errs)
+    a.close()
+
+
 def test_draw_matches_fragment_shader(builder, atm):
     """render_sky.frag:24-35 over a frame with a depth buffer; composite
     mode reproduces the dual-source blend dst = color + dst * T."""
