@@ -94,10 +94,20 @@ def render(builder, a, frames=2, px=50000):
     return rad.to_array(np.float32, (frames, px, 4)), tr.to_array(np.float32, (frames, px, 4))
 
 
+# Extents that are no powers of two (the file's rows, layers and channel
+# blocks then have odd strides).
+ODD_SIZES = dict(transmittance_mu_size=250, transmittance_r_size=60, scattering_r_size=4,
+                 scattering_mu_size=36, scattering_mu_s_size=26, scattering_nu_size=18,
+                 irradiance_mu_s_size=60, irradiance_r_size=14)
+
+
+@pytest.mark.parametrize("sizes", ["dump", "odd"])
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_save_load_round_trip(builder, tmp_path, storage):
+def test_save_load_round_trip(builder, tmp_path, storage, sizes):
     flags = abi.FB_FLAG_PRECISE | (abi.FB_FLAG_F16_STORAGE if storage == "f16" else 0)
-    p = Parameters().replace(**DUMP_SIZES, order=3, flags=flags)
+    p = Parameters().replace(**(DUMP_SIZES if sizes == "dump" else ODD_SIZES), order=3,
+                             flags=flags)
+    print(f"\n[lutio] {sizes} {storage} extents {p.scattering_extent()}", flush=True)
     a = build(builder, p)
     fp16 = storage == "f16"
     paths = a.save_all(str(tmp_path / "set_"), fp16=fp16)
