@@ -96,18 +96,26 @@ def shape_of(e):
     return (e.height, e.width, 4) if e.depth == 1 else (e.depth, e.height, e.width, 4)
 
 
+# Extents that are no powers of two (odd row and slice pitches).
+ODD_SIZES = dict(transmittance_mu_size=250, transmittance_r_size=60, scattering_r_size=4,
+                 scattering_mu_size=36, scattering_mu_s_size=26, scattering_nu_size=18,
+                 irradiance_mu_s_size=60, irradiance_r_size=14)
+
+
+@pytest.mark.parametrize("sizes", ["dump", "odd"])
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_export_reimport_matches_download(builder, storage):
+def test_export_reimport_matches_download(builder, storage, sizes):
     hip = hip_runtime()
     flags = abi.FB_FLAG_PRECISE | (abi.FB_FLAG_F16_STORAGE if storage == "f16" else 0)
-    p = Parameters().replace(**DUMP_SIZES, order=3, flags=flags)
+    p = Parameters().replace(**(DUMP_SIZES if sizes == "dump" else ODD_SIZES), order=3,
+                             flags=flags)
     a = build(builder, p)
     f16 = storage == "f16"
     want_ext = {"transmittance": p.transmittance_extent() + (1,),
                 "irradiance": p.irradiance_extent() + (1,),
                 "scattering": p.scattering_extent()}
     for name in ("transmittance", "irradiance", "scattering"):
-        print(f"export {name} {storage}", flush=True)
+        print(f"export {name} {storage} {sizes}", flush=True)
         e = a.export(name)
         try:
             stored16 = name == "scattering" and f16
