@@ -61,6 +61,12 @@ def test_build_into_matches_build(builder):
         Parameters().replace(**DUMP_SIZES, order=5, flags=precise,
                              rayleigh_scattering=[0.007, 0.015, 0.03]),  # smaller, reuses it
         Parameters().replace(**DUMP_SIZES, order=3),               # FP32 mode
+        # extents that are no powers of two (unaligned scratch offsets)
+        Parameters().replace(transmittance_mu_size=250, transmittance_r_size=60,
+                             scattering_r_size=4, scattering_mu_size=36,
+                             scattering_mu_s_size=26, scattering_nu_size=18,
+                             irradiance_mu_s_size=60, irradiance_r_size=14, order=4,
+                             flags=precise),
         Parameters().replace(flags=precise | abi.FB_FLAG_F16_STORAGE),
     ]
     for p in cases:
