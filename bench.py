@@ -588,18 +588,20 @@ def kernels_per_set(p) -> int:
     return 5 + 3 * (p.order - 1)
 
 
-def trace_summary(csv_path, last=0):
+def trace_summary(csv_path, last=0, match=""):
     """Summary of a rocprofv3 kernel trace (--kernel-trace --output-format
-    csv), over its last `last` dispatches (all when 0): the device-idle
-    fraction 1 - (union of kernel intervals) / (first start -> last end),
-    and time, count and mean duration per kernel name."""
+    csv), over its last `last` dispatches (all when 0) whose kernel name
+    contains `match`: the device-idle fraction 1 - (union of kernel
+    intervals) / (first start -> last end), and time, count and mean
+    duration per kernel name."""
     import csv
 
     rows = []
     with open(csv_path) as f:
         for row in csv.DictReader(f):
-            rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]),
-                         row["Kernel_Name"].split("(")[0]))
+            name = row["Kernel_Name"].split("(")[0]
+            if match in name:
+                rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]), name))
     rows.sort()
     if last:
         rows = rows[-last:]
@@ -653,12 +655,15 @@ def main():
                          "time per kernel) and exit (no GPU use)")
     ap.add_argument("--trace-last", type=int, default=0,
                     help="with --trace: only the last N dispatches")
+    ap.add_argument("--trace-match", default="",
+                    help="with --trace: only kernels whose name contains this text "
+                         "(e.g. '::kb_' for the batched kernels of a served run)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.trace:
-        print(json.dumps(trace_summary(args.trace, args.trace_last)))
+        print(json.dumps(trace_summary(args.trace, args.trace_last, args.trace_match)))
         return
     if args.gpus != 1:
         print(
