@@ -238,11 +238,12 @@ def test_frames_and_rays_odd_extents(builder):
     a.close()
 
 
-def test_draw_matches_fragment_shader(builder, atm):
+@pytest.mark.parametrize("w,h", [(64, 48), (67, 45)])
+def test_draw_matches_fragment_shader(builder, atm, w, h):
     """render_sky.frag:24-35 over a frame with a depth buffer; composite
-    mode reproduces the dual-source blend dst = color + dst * T."""
+    mode reproduces the dual-source blend dst = color + dst * T.  67 x 45
+    fills no block of the draw kernel evenly."""
     a, T, S, _ = atm
-    w, h = 64, 48
     # Inverse of a reverse-Z infinite perspective for a camera 2 km up looking
     # along +x (column-major, [column][row]); world units are metres
     # (render_sky.frag:27 scales by 1e-3).  Depth z unprojects to the point
